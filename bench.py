@@ -1,0 +1,167 @@
+"""Benchmark of the implicit cloth step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|plain]
+
+Workload (default): one 512x512-vertex cloth over a 4x4 square, positions
+perturbed by N(0, 1e-3*spacing) from seed 0, top edge pinned, ks 50, kd 0.25,
+mass 1, gravity -9.81, dt 1/120, CG fixed at 30 iterations, synthetic data.
+A step is one implicit step (assembly + 30 CG iterations + update) with the
+state resident in HBM.  Prints one JSON line:
+
+  value        vertex-steps/s = N * K / (time of K steps), HIP events on the
+               stream the kernels run on
+  roofline     for the dominant kernel (cg_spmv): algorithmic bytes per
+               launch (88N + 32P, the spmv share of the per-iteration model
+               160N + 32P) / its average launch time, against 8 TB/s
+  cpu_baseline the FP32 CPU restatement of the same step (oracle, one core)
+               on a bounded sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "vertex-steps/s implicit-CG step + % HBM roofline; batched fwd+adjoint vs plain"
+
+
+def spmv_bytes(N: int, P: int) -> int:
+    """Algorithmic bytes of one cg_spmv launch: symmetric blocks 24N+24P,
+    topology 8P+4N, d in / q out 24N, direction update d = r + beta d 36N."""
+    return 88 * N + 32 * P
+
+
+def step_bytes(N: int, S: int, P: int, K: int) -> int:
+    """Algorithmic bytes of one step: (61N + 12S + 32P) + K(160N + 32P) + 60N."""
+    return (61 * N + 12 * S + 32 * P) + K * (160 * N + 32 * P) + 60 * N
+
+
+def read_traffic(path: str):
+    """HBM bytes per cg_spmv launch from the committed PMC summary, or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("cg_spmv_hbm_bytes_per_launch"):
+                    return float(line.split("=")[1].split()[0])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(sc, budget_s: float = 12.0) -> dict:
+    """FP32 CPU restatement (one core) timed on whole steps of the same
+    workload until about budget_s seconds have passed."""
+    from oracle import Oracle, make_params
+
+    o = Oracle(32)
+    sysm = o.system(sc.n_vertices, sc.springs, sc.pinned)
+    prm = make_params(sc.params)
+    X = o.arr(sc.X).copy(); V = o.arr(sc.V).copy(); DV = np.zeros_like(X)
+    sysm.step(X, V, DV, prm)  # warm-up
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        sysm.step(X, V, DV, prm)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 50:
+            break
+    return {
+        "value": sc.n_vertices * n / el,
+        "unit": "vertex-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{n} steps of the same {sc.nx}x{sc.ny} workload after 1 warm-up step "
+                  f"(FP32 restatement, sequential dots, single thread, {el:.1f} s)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="large", choices=["large", "plain"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              "(see DESIGN.md for how the step would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    from opencloth_amd import scenes
+    from opencloth_amd.cloth import ImplicitCloth
+    from opencloth_amd.hiptime import Event
+
+    sc = scenes.large_scene() if args.workload == "large" else scenes.plain_scene()
+    cl = ImplicitCloth.from_scene(sc)
+    N, S, P, K = sc.n_vertices, cl.S, cl.P, sc.params.cg_max_iters
+
+    cl.step(args.warmup)
+    cl.synchronize()
+    e0, e1 = Event(), Event()
+    e0.record()
+    cl.step(args.steps)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1)
+    it, st = cl.last_status()
+    ms_per_step = ms / args.steps
+
+    prof = cl.profile_steps(min(args.steps, 20))
+    spmv_ms, spmv_n = prof["cg_spmv"]
+    spmv_avg_s = spmv_ms / max(spmv_n, 1) * 1e-3
+    achieved = spmv_bytes(N, P) / spmv_avg_s / 1e9
+    traffic = read_traffic(os.path.join(ROOT, "profiles", "cg_spmv_512x512_pmc.txt"))
+
+    out = {
+        "metric": METRIC,
+        "value": N * args.steps / (ms * 1e-3),
+        "unit": "vertex-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"implicit step, {sc.nx}x{sc.ny} cloth, "
+                        + ("top edge pinned, CG fixed 30 iterations, dt 1/120"
+                           if args.workload == "large" else "2 corners pinned, CG <=10 it tol 1e-3, dt 1/60"),
+            "vertices": N, "springs": S, "pairs": P, "cg_iters": K,
+            "step_algorithmic_bytes": step_bytes(N, S, P, K),
+            "step_roofline_frac": step_bytes(N, S, P, K) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "last_cg_iters": it, "last_status": st,
+            "kernel_ms_per_step": {k: v[0] / max(min(args.steps, 20), 1) for k, v in prof.items()},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "cg_spmv",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": spmv_bytes(N, P),
+            "avg_launch_us": spmv_avg_s * 1e6,
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sc)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

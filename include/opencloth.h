@@ -1,0 +1,168 @@
+/*
+ * opencloth.h -- C ABI of the MI355X implicit mass-spring cloth library.
+ *
+ * The reference (mmmovania/opencloth) has no library interface: its demos
+ * keep the cloth in file-scope globals and call free `void` functions from
+ * the GLUT idle loop.  Each entry point below replaces one of those
+ * functions for a C or C++ caller; the comment names the function it
+ * replaces (paths relative to the reference repository):
+ *
+ *   IE  = OpenCloth_ImplicitEuler/OpenCloth_ImplicitEuler/main.cpp
+ *   BW  = OpenCloth_Implicit/OpenCloth_Implicit/main.cpp
+ *   VL  = OpenCloth_Verlet/OpenCloth_Verlet/main.cpp
+ *   PBD = OpenCloth_PositionBasedDynamics/OpenCloth_PositionBasedDynamics/main.cpp
+ *
+ * Conventions
+ *   - Host arrays are AoS float3 with a 12-byte stride, the layout of the
+ *     reference's vector<glm::vec3> (IE:279-281); the caller owns them and
+ *     the library copies in and out.  Device storage is internal SoA.
+ *   - Springs use the reference's record (IE:68-73).  Kd keeps the
+ *     reference's sign: negative values damp (IE:310-312).
+ *   - Every function returns an oc_status; nothing throws across the ABI.
+ *   - A handle is used by one thread at a time.  Work is ordered on the
+ *     handle's HIP stream (oc_cloth_set_stream); functions that return host
+ *     data synchronise that stream.
+ *   - Results are deterministic: forces and Jacobian blocks are gathered per
+ *     vertex with no atomics and every reduction has a fixed order.
+ */
+#ifndef OPENCLOTH_H
+#define OPENCLOTH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+typedef enum {
+    OC_OK = 0,
+    OC_ERR_INVALID_ARGUMENT = -1,
+    OC_ERR_UNSUPPORTED_TOPOLOGY = -2, /* a spring spans more than the tile halo */
+    OC_ERR_DEVICE = -3,               /* a HIP call failed */
+    OC_ERR_OUT_OF_MEMORY = -4,
+    OC_ERR_NO_DEVICE = -5
+} oc_status;
+
+/* solver status bits reported per step (per instance in batches) */
+#define OC_SOLVE_NOT_CONVERGED 1 /* tolerance not met within the iteration cap */
+#define OC_SOLVE_BREAKDOWN 2     /* d.q <= 0 or non-finite in CG */
+#define OC_SOLVE_NONFINITE 4     /* NaN or Inf in the state */
+
+/* CG stopping tests */
+#define OC_TOL_ABSOLUTE 0 /* continue while r.r > tol^2 (IE:238) */
+#define OC_TOL_RELATIVE 1 /* continue while r.r > tol^2 * r0.r0 (BW:292) */
+#define OC_TOL_NONE 2     /* run exactly cg_max_iters iterations */
+
+/* oc_params.flags */
+#define OC_FLAG_GROUND_CLAMP 1 /* clamp y < 0 to 0 after the update (IE:695-697) */
+
+/* Spring record, identical to the reference's `struct Spring` (IE:68-73). */
+typedef struct {
+    int32_t p1, p2;
+    float rest_length;
+    float Ks, Kd;
+    int32_t type;
+} oc_spring;
+
+/* Scalars of the implicit step.  The reference keeps them in globals:
+ * mass (IE:314), gravity (IE:313, an acceleration here), timeStep (IE:316),
+ * EPS and i_max (IE:75-77). */
+typedef struct {
+    float mass;
+    float gravity[3];
+    float dt;
+    int32_t cg_max_iters;
+    float cg_tol;
+    int32_t cg_tol_mode; /* OC_TOL_* */
+    int32_t warm_start;  /* nonzero: CG starts from the previous step's dv (IE:691, BW:788) */
+    int32_t flags;       /* OC_FLAG_* */
+} oc_params;
+
+/* ---- library ----------------------------------------------------------- */
+/* Version string of the library. */
+const char* oc_version(void);
+/* Text of the last error on this thread (empty if none). */
+const char* oc_last_error(void);
+/* Number of visible HIP devices (0 when none). */
+int oc_device_count(void);
+
+/* ---- implicit cloth (the hot path) ------------------------------------- */
+typedef struct oc_cloth_s* oc_cloth;
+
+/* Create a cloth on the current HIP device.  Vertices form an nx*ny grid,
+ * vertex k = j*nx + i (IE:462-466).  x0/v0: host AoS 3*nx*ny floats (v0 may
+ * be NULL for rest).  springs: the reference spring list (IE:496-530 or any
+ * list whose springs join vertices at most 2 rows and 2 columns apart).
+ * pinned: nx*ny bytes, nonzero = fixed vertex (IE:652 pins {0, numX}); NULL
+ * pins nothing.  Replaces InitGL's state set-up (IE:434-546). */
+int oc_cloth_create(int32_t nx, int32_t ny, const float* x0, const float* v0,
+                    const oc_spring* springs, int32_t n_springs, const uint8_t* pinned,
+                    const oc_params* params, oc_cloth* out);
+/* Free a cloth.  Replaces OnShutdown (IE:635-643). */
+int oc_cloth_destroy(oc_cloth cloth);
+
+/* Order all later work on `hip_stream` (a hipStream_t; NULL = default stream). */
+int oc_cloth_set_stream(oc_cloth cloth, void* hip_stream);
+int oc_cloth_set_params(oc_cloth cloth, const oc_params* params);
+int oc_cloth_get_params(oc_cloth cloth, oc_params* params);
+/* Copy positions/velocities in (either may be NULL).  The reference's mouse
+ * drag writes X and V between steps (IE:405-410). */
+int oc_cloth_set_state(oc_cloth cloth, const float* x, const float* v);
+/* Copy positions/velocities out (either may be NULL); synchronises. */
+int oc_cloth_get_state(oc_cloth cloth, float* x, float* v);
+/* Copy the CG solution dv out (host AoS); synchronises. */
+int oc_cloth_get_dv(oc_cloth cloth, float* dv);
+/* Set the CG start value dv (host AoS, may be NULL for zero). */
+int oc_cloth_set_dv(oc_cloth cloth, const float* dv);
+
+/* Spring forces plus gravity on every free vertex, zero on pinned ones.
+ * Replaces ComputeForces (IE:645-682).  f_out: host AoS 3N (synchronises),
+ * or NULL to leave the result on the device. */
+int oc_cloth_compute_forces(oc_cloth cloth, float* f_out);
+
+/* Assemble (M - h df/dv - h^2 df/dx) dv = h (f + h df/dx v) at the current
+ * state with h = dt, and set up CG from the start value.  The first half of
+ * IntegrateImplicit (IE:684-689, in the dv form of BW:775-786). */
+int oc_cloth_assemble(oc_cloth cloth, float dt);
+/* Solve the assembled system by filtered CG (pinned rows and columns
+ * removed).  Replaces SolveConjugateGradient (IE:227-249).  iters/status
+ * (either may be NULL) receive the iteration count and OC_SOLVE_* bits;
+ * synchronises when either is given. */
+int oc_cloth_solve(oc_cloth cloth, int32_t* iters, int32_t* status);
+/* v += dv, x += dt*v, optional ground clamp (IE:693-699). */
+int oc_cloth_update(oc_cloth cloth, float dt);
+/* assemble + solve + update.  Replaces IntegrateImplicit (IE:684-700). */
+int oc_cloth_integrate_implicit(oc_cloth cloth, float dt);
+/* nsteps implicit steps with the handle's parameters, fully on the device.
+ * Replaces StepPhysics (IE:774-779) for the implicit integrator; the
+ * reference's post-step collision and Provot passes are separate calls. */
+int oc_cloth_step(oc_cloth cloth, int32_t nsteps);
+/* Wait for the handle's stream. */
+int oc_cloth_synchronize(oc_cloth cloth);
+/* Status of the last solve: CG iterations and OC_SOLVE_* bits; synchronises. */
+int oc_cloth_last_status(oc_cloth cloth, int32_t* iters, int32_t* status);
+
+/* Measurement hook (no counterpart in the reference, whose only timing is a
+ * frame counter, IE:562-584): runs nsteps like oc_cloth_step with a HIP
+ * event after every kernel launch and returns, per kernel class
+ * (0 assemble, 1 cg_spmv, 2 cg_axpy, 3 cg_finish, 4 update), the summed
+ * milliseconds between consecutive events and the launch counts.
+ * ms[5], launches[5]; synchronises. */
+int oc_cloth_profile_steps(oc_cloth cloth, int32_t nsteps, float* ms, int32_t* launches);
+
+/* Introspection of the assembled system (for tests and tools).
+ * Pairs are the unique unordered vertex pairs joined by springs, numbered by
+ * their first spring; a[p] < b[p]. */
+int oc_cloth_num_pairs(oc_cloth cloth, int32_t* n_pairs);
+int oc_cloth_get_pairs(oc_cloth cloth, int32_t* a, int32_t* b);
+/* Off-diagonal blocks C_ab = A_ab of every pair as 6 floats
+ * (xx,yy,zz,xy,xz,yz), and the right-hand side h(f + h df/dx v) with pinned
+ * rows zero (host AoS); either may be NULL; synchronises. */
+int oc_cloth_get_system(oc_cloth cloth, float* blocks, float* rhs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPENCLOTH_H */

@@ -1,0 +1,95 @@
+"""ctypes binding of libopencloth.so (the C ABI in include/opencloth.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libopencloth.so")
+
+OC_OK = 0
+OC_ERR_INVALID_ARGUMENT = -1
+OC_ERR_UNSUPPORTED_TOPOLOGY = -2
+OC_ERR_DEVICE = -3
+OC_ERR_OUT_OF_MEMORY = -4
+OC_ERR_NO_DEVICE = -5
+
+SOLVE_NOT_CONVERGED, SOLVE_BREAKDOWN, SOLVE_NONFINITE = 1, 2, 4
+
+# Every symbol include/opencloth.h declares, with (restype, argtypes).
+_P = C.c_void_p
+_I = C.c_int32
+_F = C.c_float
+SIGNATURES = {
+    "oc_version": (C.c_char_p, []),
+    "oc_last_error": (C.c_char_p, []),
+    "oc_device_count": (C.c_int, []),
+    "oc_cloth_create": (C.c_int, [_I, _I, _P, _P, _P, _I, _P, _P, _P]),
+    "oc_cloth_destroy": (C.c_int, [_P]),
+    "oc_cloth_set_stream": (C.c_int, [_P, _P]),
+    "oc_cloth_set_params": (C.c_int, [_P, _P]),
+    "oc_cloth_get_params": (C.c_int, [_P, _P]),
+    "oc_cloth_set_state": (C.c_int, [_P, _P, _P]),
+    "oc_cloth_get_state": (C.c_int, [_P, _P, _P]),
+    "oc_cloth_get_dv": (C.c_int, [_P, _P]),
+    "oc_cloth_set_dv": (C.c_int, [_P, _P]),
+    "oc_cloth_compute_forces": (C.c_int, [_P, _P]),
+    "oc_cloth_assemble": (C.c_int, [_P, _F]),
+    "oc_cloth_solve": (C.c_int, [_P, _P, _P]),
+    "oc_cloth_update": (C.c_int, [_P, _F]),
+    "oc_cloth_integrate_implicit": (C.c_int, [_P, _F]),
+    "oc_cloth_step": (C.c_int, [_P, _I]),
+    "oc_cloth_synchronize": (C.c_int, [_P]),
+    "oc_cloth_last_status": (C.c_int, [_P, _P, _P]),
+    "oc_cloth_profile_steps": (C.c_int, [_P, _I, _P, _P]),
+    "oc_cloth_num_pairs": (C.c_int, [_P, _P]),
+    "oc_cloth_get_pairs": (C.c_int, [_P, _P, _P]),
+    "oc_cloth_get_system": (C.c_int, [_P, _P, _P]),
+}
+
+
+class OcParams(C.Structure):
+    """oc_params (include/opencloth.h)."""
+
+    _fields_ = [
+        ("mass", C.c_float), ("gravity", C.c_float * 3), ("dt", C.c_float),
+        ("cg_max_iters", C.c_int32), ("cg_tol", C.c_float), ("cg_tol_mode", C.c_int32),
+        ("warm_start", C.c_int32), ("flags", C.c_int32),
+    ]
+
+
+class OpenClothError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"opencloth error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OpenClothError(-100, f"{LIB_PATH} is missing: run `python -m opencloth_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != OC_OK:
+        raise OpenClothError(rc, lib().oc_last_error().decode(errors="replace"))
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)

@@ -1,0 +1,170 @@
+"""Python mirror of the reference's implicit-step interface over the C ABI.
+
+The reference drives one cloth held in globals through free functions
+(OpenCloth_ImplicitEuler/OpenCloth_ImplicitEuler/main.cpp):
+ComputeForces (:645), IntegrateImplicit (:684), SolveConjugateGradient
+(:227), StepPhysics (:774).  `ImplicitCloth` keeps the same operations on a
+handle whose state lives in device memory.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import OcParams, check, lib, ptr
+from .scenes import SPRING_DTYPE, Scene, StepParams
+
+
+def to_oc_params(p: StepParams) -> OcParams:
+    q = OcParams()
+    q.mass = p.mass
+    q.gravity[0], q.gravity[1], q.gravity[2] = p.gravity
+    q.dt = p.dt
+    q.cg_max_iters = p.cg_max_iters
+    q.cg_tol = p.cg_tol
+    q.cg_tol_mode = p.cg_tol_mode
+    q.warm_start = 1 if p.warm_start else 0
+    q.flags = 1 if p.ground_clamp else 0
+    return q
+
+
+def _vec(a, n):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.shape != (n, 3):
+        raise ValueError(f"expected shape ({n}, 3), got {a.shape}")
+    return a
+
+
+class ImplicitCloth:
+    """One cloth on the current HIP device (oc_cloth)."""
+
+    def __init__(self, nx, ny, X, springs, pinned=None, params: StepParams | None = None, V=None):
+        self.nx, self.ny = int(nx), int(ny)
+        self.N = self.nx * self.ny
+        self.params = params or StepParams()
+        X = _vec(X, self.N)
+        V = None if V is None else _vec(V, self.N)
+        springs = np.ascontiguousarray(springs, dtype=SPRING_DTYPE)
+        pins = None if pinned is None else np.ascontiguousarray(pinned, np.uint8)
+        if pins is not None and pins.shape != (self.N,):
+            raise ValueError("pinned must have one entry per vertex")
+        self._p = to_oc_params(self.params)
+        h = C.c_void_p()
+        check(lib().oc_cloth_create(self.nx, self.ny, ptr(X), ptr(V), ptr(springs), len(springs),
+                                    ptr(pins), C.byref(self._p), C.byref(h)))
+        self.h = h
+        self.S = len(springs)
+        n = C.c_int32()
+        check(lib().oc_cloth_num_pairs(self.h, C.byref(n)))
+        self.P = n.value
+
+    @classmethod
+    def from_scene(cls, sc: Scene) -> "ImplicitCloth":
+        return cls(sc.nx, sc.ny, sc.X, sc.springs, sc.pinned, sc.params, sc.V)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().oc_cloth_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- configuration -------------------------------------------------------
+    def set_stream(self, stream_handle: int | None):
+        check(lib().oc_cloth_set_stream(self.h, C.c_void_p(stream_handle or 0)))
+
+    def set_params(self, params: StepParams):
+        self.params = params
+        self._p = to_oc_params(params)
+        check(lib().oc_cloth_set_params(self.h, C.byref(self._p)))
+
+    # -- state ---------------------------------------------------------------
+    def set_state(self, X=None, V=None):
+        Xa = None if X is None else _vec(X, self.N)
+        Va = None if V is None else _vec(V, self.N)
+        check(lib().oc_cloth_set_state(self.h, ptr(Xa), ptr(Va)))
+
+    def get_state(self):
+        X = np.zeros((self.N, 3), np.float32)
+        V = np.zeros((self.N, 3), np.float32)
+        check(lib().oc_cloth_get_state(self.h, ptr(X), ptr(V)))
+        return X, V
+
+    @property
+    def X(self):
+        return self.get_state()[0]
+
+    @property
+    def V(self):
+        return self.get_state()[1]
+
+    def get_dv(self):
+        dv = np.zeros((self.N, 3), np.float32)
+        check(lib().oc_cloth_get_dv(self.h, ptr(dv)))
+        return dv
+
+    def set_dv(self, dv=None):
+        a = None if dv is None else _vec(dv, self.N)
+        check(lib().oc_cloth_set_dv(self.h, ptr(a)))
+
+    # -- the reference's operations ------------------------------------------
+    def compute_forces(self) -> np.ndarray:
+        """ComputeForces (IE:645-682): gravity plus spring forces, zero on pins."""
+        F = np.zeros((self.N, 3), np.float32)
+        check(lib().oc_cloth_compute_forces(self.h, ptr(F)))
+        return F
+
+    def assemble(self, dt=None):
+        check(lib().oc_cloth_assemble(self.h, self.params.dt if dt is None else dt))
+
+    def solve_conjugate_gradient(self):
+        """SolveConjugateGradient (IE:227-249) on the assembled system;
+        returns (iterations, status bits)."""
+        it = C.c_int32(); st = C.c_int32()
+        check(lib().oc_cloth_solve(self.h, C.byref(it), C.byref(st)))
+        return it.value, st.value
+
+    def update(self, dt=None):
+        check(lib().oc_cloth_update(self.h, self.params.dt if dt is None else dt))
+
+    def integrate_implicit(self, dt=None):
+        """IntegrateImplicit (IE:684-700)."""
+        check(lib().oc_cloth_integrate_implicit(self.h, self.params.dt if dt is None else dt))
+
+    def step(self, nsteps: int = 1):
+        """StepPhysics (IE:774-779) of the implicit integrator, nsteps times."""
+        check(lib().oc_cloth_step(self.h, int(nsteps)))
+
+    def synchronize(self):
+        check(lib().oc_cloth_synchronize(self.h))
+
+    KERNEL_CLASSES = ("assemble", "cg_spmv", "cg_axpy", "cg_finish", "update")
+
+    def profile_steps(self, nsteps: int) -> dict:
+        """nsteps steps with an event after every launch: {kernel: (ms, launches)}."""
+        ms = (C.c_float * 5)(); n = (C.c_int32 * 5)()
+        check(lib().oc_cloth_profile_steps(self.h, int(nsteps), ms, n))
+        return {k: (ms[i], n[i]) for i, k in enumerate(self.KERNEL_CLASSES)}
+
+    def last_status(self):
+        it = C.c_int32(); st = C.c_int32()
+        check(lib().oc_cloth_last_status(self.h, C.byref(it), C.byref(st)))
+        return it.value, st.value
+
+    # -- introspection -------------------------------------------------------
+    def pairs(self):
+        a = np.zeros(self.P, np.int32); b = np.zeros(self.P, np.int32)
+        check(lib().oc_cloth_get_pairs(self.h, ptr(a), ptr(b)))
+        return a, b
+
+    def system(self):
+        """(pair blocks [P,6], rhs [N,3]) of the last assembly."""
+        Cb = np.zeros((self.P, 6), np.float32); b = np.zeros((self.N, 3), np.float32)
+        check(lib().oc_cloth_get_system(self.h, ptr(Cb), ptr(b)))
+        return Cb, b

@@ -1,0 +1,472 @@
+// C ABI of the implicit cloth (include/opencloth.h): handle lifetime, host
+// <-> device copies (AoS float3 on the host, SoA planes on the device) and
+// the launch sequence of one step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "oc_internal.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define OC_HIP(call)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(OC_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace
+
+struct oc_cloth_s {
+    oc::GridTopology topo;
+    oc_params prm{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int max_it_alloc = 0;
+    void* arena = nullptr;        // vertex vectors, topology, blocks
+    void* scratch = nullptr;      // CG partials and scalars
+    int* ctl_host = nullptr;      // pinned copy of ctl
+    oc::DevCloth dc{};
+    bool assembled = false;
+};
+
+namespace {
+
+oc::StepScalars scalars(const oc_cloth_s* c, float dt) {
+    oc::StepScalars s;
+    s.mass = c->prm.mass;
+    s.gx = c->prm.gravity[0];
+    s.gy = c->prm.gravity[1];
+    s.gz = c->prm.gravity[2];
+    s.h = dt;
+    s.hh = dt * dt;
+    s.tol2 = c->prm.cg_tol * c->prm.cg_tol;
+    s.max_it = c->prm.cg_max_iters;
+    s.tol_mode = c->prm.cg_tol_mode;
+    s.warm = c->prm.warm_start ? 1 : 0;
+    s.flags = c->prm.flags;
+    return s;
+}
+
+int check_params(const oc_params* p) {
+    if (!p) return fail(OC_ERR_INVALID_ARGUMENT, "params is NULL");
+    if (!(p->mass > 0.0f) || !std::isfinite(p->mass))
+        return fail(OC_ERR_INVALID_ARGUMENT, "mass must be positive");
+    if (!(p->dt > 0.0f) || !std::isfinite(p->dt))
+        return fail(OC_ERR_INVALID_ARGUMENT, "dt must be positive");
+    if (p->cg_max_iters < 0 || p->cg_max_iters > 100000)
+        return fail(OC_ERR_INVALID_ARGUMENT, "cg_max_iters out of range");
+    if (p->cg_tol_mode < 0 || p->cg_tol_mode > 2)
+        return fail(OC_ERR_INVALID_ARGUMENT, "unknown cg_tol_mode");
+    if (!(p->cg_tol >= 0.0f)) return fail(OC_ERR_INVALID_ARGUMENT, "cg_tol must be >= 0");
+    return OC_OK;
+}
+
+// (Re)allocate the CG partials for the current iteration cap.
+int ensure_scratch(oc_cloth_s* c) {
+    const int need = std::max(c->prm.cg_max_iters, 1);
+    if (c->scratch && need <= c->max_it_alloc) return OC_OK;
+    if (c->scratch) {
+        OC_HIP(hipStreamSynchronize(c->stream));
+        OC_HIP(hipFree(c->scratch));
+        c->scratch = nullptr;
+    }
+    const size_t nt = c->dc.nt;
+    const size_t bytes = align_up(sizeof(float) * (need + 1) * nt) +
+                         align_up(sizeof(float) * need * nt) + align_up(sizeof(float) * (need + 2)) +
+                         align_up(sizeof(int) * 8);
+    if (hipMalloc(&c->scratch, bytes) != hipSuccess)
+        return fail(OC_ERR_OUT_OF_MEMORY, "hipMalloc of CG scratch failed");
+    char* p = (char*)c->scratch;
+    c->dc.del_part = (float*)p;
+    p += align_up(sizeof(float) * (need + 1) * nt);
+    c->dc.sig_part = (float*)p;
+    p += align_up(sizeof(float) * need * nt);
+    c->dc.scal = (float*)p;
+    p += align_up(sizeof(float) * (need + 2));
+    c->dc.ctl = (int*)p;
+    OC_HIP(hipMemsetAsync(c->scratch, 0, bytes, c->stream));
+    c->max_it_alloc = need;
+    return OC_OK;
+}
+
+void aos_to_soa(const float* a, int N, std::vector<float>& out) {
+    out.resize(3 * (size_t)N);
+    for (int i = 0; i < N; i++)
+        for (int k = 0; k < 3; k++) out[(size_t)k * N + i] = a[3 * (size_t)i + k];
+}
+void soa_to_aos(const std::vector<float>& s, int N, float* a) {
+    for (int i = 0; i < N; i++)
+        for (int k = 0; k < 3; k++) a[3 * (size_t)i + k] = s[(size_t)k * N + i];
+}
+
+int upload_vec(oc_cloth_s* c, float* dst, const float* host_aos) {
+    std::vector<float> tmp;
+    if (host_aos)
+        aos_to_soa(host_aos, c->dc.N, tmp);
+    else
+        tmp.assign(3 * (size_t)c->dc.N, 0.0f);
+    OC_HIP(hipMemcpyAsync(dst, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice,
+                          c->stream));
+    OC_HIP(hipStreamSynchronize(c->stream));
+    return OC_OK;
+}
+
+int download_vec(oc_cloth_s* c, const float* src, float* host_aos) {
+    std::vector<float> tmp(3 * (size_t)c->dc.N);
+    OC_HIP(hipMemcpyAsync(tmp.data(), src, tmp.size() * sizeof(float), hipMemcpyDeviceToHost,
+                          c->stream));
+    OC_HIP(hipStreamSynchronize(c->stream));
+    soa_to_aos(tmp, c->dc.N, host_aos);
+    return OC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* oc_version(void) { return "opencloth_amd 0.1 (gfx950)"; }
+
+const char* oc_last_error(void) { return g_err.c_str(); }
+
+int oc_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int oc_cloth_create(int32_t nx, int32_t ny, const float* x0, const float* v0,
+                    const oc_spring* springs, int32_t n_springs, const uint8_t* pinned,
+                    const oc_params* params, oc_cloth* out) {
+    g_err.clear();
+    if (!out) return fail(OC_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (!x0) return fail(OC_ERR_INVALID_ARGUMENT, "x0 is NULL");
+    int rc = check_params(params);
+    if (rc != OC_OK) return rc;
+    if (oc_device_count() < 1) return fail(OC_ERR_NO_DEVICE, "no HIP device");
+
+    oc_cloth_s* c = new (std::nothrow) oc_cloth_s();
+    if (!c) return fail(OC_ERR_OUT_OF_MEMORY, "host allocation failed");
+    std::string err;
+    rc = oc::build_grid_topology(nx, ny, springs, n_springs, &c->topo, &err);
+    if (rc != OC_OK) {
+        delete c;
+        return fail(rc, err);
+    }
+    c->prm = *params;
+    if (hipGetDevice(&c->device) != hipSuccess) {
+        delete c;
+        return fail(OC_ERR_DEVICE, "hipGetDevice failed");
+    }
+    const oc::GridTopology& t = c->topo;
+    const size_t N = t.N, S = std::max(t.S, 1);
+    oc::DevCloth& d = c->dc;
+    d.nx = t.nx;
+    d.ny = t.ny;
+    d.N = t.N;
+    d.ntx = (t.nx + oc::kTileX - 1) / oc::kTileX;
+    d.nty = (t.ny + oc::kTileY - 1) / oc::kTileY;
+    d.nt = d.ntx * d.nty;
+    d.wa = t.wa;
+    d.wp = t.wp;
+
+    const size_t vec = align_up(sizeof(float) * 3 * N);
+    const size_t blocks = align_up(sizeof(float) * 6 * N * std::max(t.n_planes, 1));
+    const size_t ainc_s = align_up(sizeof(int32_t) * std::max<size_t>(t.ainc_spring.size(), 1));
+    const size_t ainc_m = align_up(sizeof(uint16_t) * std::max<size_t>(t.ainc_meta.size(), 1));
+    const size_t pair_m = align_up(sizeof(uint16_t) * std::max<size_t>(t.pair_meta.size(), 1));
+    const size_t sp = align_up(sizeof(float) * S);
+    const size_t pin = align_up(N);
+    const size_t total = 9 * vec + blocks + ainc_s + ainc_m + pair_m + 3 * sp + pin;
+    if (hipMalloc(&c->arena, total) != hipSuccess) {
+        delete c;
+        return fail(OC_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(total) + " bytes failed");
+    }
+    char* p = (char*)c->arena;
+    auto take = [&](size_t n) {
+        char* q = p;
+        p += n;
+        return q;
+    };
+    d.x = (float*)take(vec);
+    d.v = (float*)take(vec);
+    d.dv = (float*)take(vec);
+    d.r = (float*)take(vec);
+    d.d0 = (float*)take(vec);
+    d.d1 = (float*)take(vec);
+    d.q = (float*)take(vec);
+    d.b = (float*)take(vec);
+    d.f = (float*)take(vec);
+    d.blocks = (float*)take(blocks);
+    d.ainc_s = (const int32_t*)take(ainc_s);
+    d.ainc_m = (const uint16_t*)take(ainc_m);
+    d.pair_m = (const uint16_t*)take(pair_m);
+    d.L0 = (const float*)take(sp);
+    d.ks = (const float*)take(sp);
+    d.kd = (const float*)take(sp);
+    d.pin = (const uint8_t*)take(pin);
+
+    auto cleanup = [&](int code) {
+        (void)hipFree(c->arena);
+        if (c->scratch) (void)hipFree(c->scratch);
+        std::string keep = g_err;
+        delete c;
+        g_err = keep;
+        return code;
+    };
+    if (hipMemset(c->arena, 0, total) != hipSuccess)
+        return cleanup(fail(OC_ERR_DEVICE, "hipMemset failed"));
+    std::vector<float> L0(S, 0.0f), ks(S, 0.0f), kd(S, 0.0f);
+    for (int s = 0; s < t.S; s++) {
+        L0[s] = springs[s].rest_length;
+        ks[s] = springs[s].Ks;
+        kd[s] = springs[s].Kd;
+    }
+    std::vector<uint8_t> pins(N, 0);
+    if (pinned)
+        for (size_t i = 0; i < N; i++) pins[i] = pinned[i] ? 1 : 0;
+    bool ok = hipMemcpy((void*)d.ainc_s, t.ainc_spring.data(), t.ainc_spring.size() * 4,
+                        hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy((void*)d.ainc_m, t.ainc_meta.data(), t.ainc_meta.size() * 2,
+                        hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy((void*)d.pair_m, t.pair_meta.data(), t.pair_meta.size() * 2,
+                        hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy((void*)d.L0, L0.data(), S * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy((void*)d.ks, ks.data(), S * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy((void*)d.kd, kd.data(), S * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy((void*)d.pin, pins.data(), N, hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) return cleanup(fail(OC_ERR_DEVICE, "upload of topology failed"));
+    if (hipHostMalloc((void**)&c->ctl_host, sizeof(int) * 8) != hipSuccess)
+        return cleanup(fail(OC_ERR_OUT_OF_MEMORY, "hipHostMalloc failed"));
+    if ((rc = ensure_scratch(c)) != OC_OK) return cleanup(rc);
+    if ((rc = upload_vec(c, d.x, x0)) != OC_OK) return cleanup(rc);
+    if ((rc = upload_vec(c, d.v, v0)) != OC_OK) return cleanup(rc);
+    *out = c;
+    return OC_OK;
+}
+
+int oc_cloth_destroy(oc_cloth c) {
+    if (!c) return OC_OK;
+    (void)hipStreamSynchronize(c->stream);
+    if (c->arena) (void)hipFree(c->arena);
+    if (c->scratch) (void)hipFree(c->scratch);
+    if (c->ctl_host) (void)hipHostFree(c->ctl_host);
+    delete c;
+    return OC_OK;
+}
+
+int oc_cloth_set_stream(oc_cloth c, void* stream) {
+    if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
+    c->stream = (hipStream_t)stream;
+    return OC_OK;
+}
+
+int oc_cloth_set_params(oc_cloth c, const oc_params* p) {
+    if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
+    int rc = check_params(p);
+    if (rc != OC_OK) return rc;
+    c->prm = *p;
+    return ensure_scratch(c);
+}
+
+int oc_cloth_get_params(oc_cloth c, oc_params* p) {
+    if (!c || !p) return fail(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    *p = c->prm;
+    return OC_OK;
+}
+
+int oc_cloth_set_state(oc_cloth c, const float* x, const float* v) {
+    if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
+    int rc;
+    if (x && (rc = upload_vec(c, c->dc.x, x)) != OC_OK) return rc;
+    if (v && (rc = upload_vec(c, c->dc.v, v)) != OC_OK) return rc;
+    c->assembled = false;
+    return OC_OK;
+}
+
+int oc_cloth_get_state(oc_cloth c, float* x, float* v) {
+    if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
+    int rc;
+    if (x && (rc = download_vec(c, c->dc.x, x)) != OC_OK) return rc;
+    if (v && (rc = download_vec(c, c->dc.v, v)) != OC_OK) return rc;
+    return OC_OK;
+}
+
+int oc_cloth_get_dv(oc_cloth c, float* dv) {
+    if (!c || !dv) return fail(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    return download_vec(c, c->dc.dv, dv);
+}
+
+int oc_cloth_set_dv(oc_cloth c, const float* dv) {
+    if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
+    return upload_vec(c, c->dc.dv, dv);
+}
+
+int oc_cloth_compute_forces(oc_cloth c, float* f_out) {
+    if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
+    OC_HIP(oc::launch_forces(c->dc, scalars(c, c->prm.dt), c->stream));
+    if (f_out) return download_vec(c, c->dc.f, f_out);
+    return OC_OK;
+}
+
+int oc_cloth_assemble(oc_cloth c, float dt) {
+    if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
+    if (!(dt > 0.0f)) return fail(OC_ERR_INVALID_ARGUMENT, "dt must be positive");
+    OC_HIP(oc::launch_assemble(c->dc, scalars(c, dt), c->stream));
+    c->assembled = true;
+    return OC_OK;
+}
+
+int oc_cloth_solve(oc_cloth c, int32_t* iters, int32_t* status) {
+    if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
+    if (!c->assembled) return fail(OC_ERR_INVALID_ARGUMENT, "solve before assemble");
+    OC_HIP(oc::launch_cg(c->dc, scalars(c, c->prm.dt), c->stream));
+    if (iters || status) {
+        OC_HIP(hipMemcpyAsync(c->ctl_host, c->dc.ctl, sizeof(int) * 4, hipMemcpyDeviceToHost,
+                              c->stream));
+        OC_HIP(hipStreamSynchronize(c->stream));
+        if (iters) *iters = c->ctl_host[1];
+        if (status) *status = c->ctl_host[2];
+    }
+    return OC_OK;
+}
+
+int oc_cloth_update(oc_cloth c, float dt) {
+    if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
+    if (!(dt > 0.0f)) return fail(OC_ERR_INVALID_ARGUMENT, "dt must be positive");
+    OC_HIP(oc::launch_update(c->dc, scalars(c, dt), c->stream));
+    c->assembled = false;
+    return OC_OK;
+}
+
+int oc_cloth_integrate_implicit(oc_cloth c, float dt) {
+    if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
+    if (!(dt > 0.0f)) return fail(OC_ERR_INVALID_ARGUMENT, "dt must be positive");
+    const oc::StepScalars s = scalars(c, dt);
+    OC_HIP(oc::launch_assemble(c->dc, s, c->stream));
+    OC_HIP(oc::launch_cg(c->dc, s, c->stream));
+    OC_HIP(oc::launch_update(c->dc, s, c->stream));
+    c->assembled = false;
+    return OC_OK;
+}
+
+int oc_cloth_step(oc_cloth c, int32_t nsteps) {
+    if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
+    if (nsteps < 0) return fail(OC_ERR_INVALID_ARGUMENT, "nsteps < 0");
+    const oc::StepScalars s = scalars(c, c->prm.dt);
+    for (int k = 0; k < nsteps; k++) {
+        OC_HIP(oc::launch_assemble(c->dc, s, c->stream));
+        OC_HIP(oc::launch_cg(c->dc, s, c->stream));
+        OC_HIP(oc::launch_update(c->dc, s, c->stream));
+    }
+    c->assembled = false;
+    return OC_OK;
+}
+
+int oc_cloth_profile_steps(oc_cloth c, int32_t nsteps, float* ms, int32_t* launches) {
+    if (!c || !ms || !launches) return fail(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (nsteps < 0) return fail(OC_ERR_INVALID_ARGUMENT, "nsteps < 0");
+    const oc::StepScalars s = scalars(c, c->prm.dt);
+    const int per_step = 2 * s.max_it + 3;
+    std::vector<hipEvent_t> ev(per_step + 1);
+    std::vector<int> cls(per_step);
+    for (auto& e : ev) OC_HIP(hipEventCreate(&e));
+    for (int k = 0; k < 5; k++) {
+        ms[k] = 0.0f;
+        launches[k] = 0;
+    }
+    int rc = OC_OK;
+    for (int step = 0; step < nsteps && rc == OC_OK; step++) {
+        int n = 0;
+        auto rec = [&](hipError_t e, int kind) {
+            if (e != hipSuccess) return false;
+            cls[n] = kind;
+            return hipEventRecord(ev[++n], c->stream) == hipSuccess;
+        };
+        bool ok = hipEventRecord(ev[0], c->stream) == hipSuccess &&
+                  rec(oc::launch_assemble(c->dc, s, c->stream), 0);
+        for (int k = 0; ok && k < s.max_it; k++)
+            ok = rec(oc::launch_cg_spmv(c->dc, s, k, c->stream), 1) &&
+                 rec(oc::launch_cg_axpy(c->dc, s, k, c->stream), 2);
+        ok = ok && rec(oc::launch_cg_finish(c->dc, s, c->stream), 3) &&
+             rec(oc::launch_update(c->dc, s, c->stream), 4);
+        if (!ok || hipEventSynchronize(ev[n]) != hipSuccess) {
+            rc = fail(OC_ERR_DEVICE, "profiled launch failed");
+            break;
+        }
+        for (int k = 0; k < n; k++) {
+            float t = 0.0f;
+            (void)hipEventElapsedTime(&t, ev[k], ev[k + 1]);
+            ms[cls[k]] += t;
+            launches[cls[k]]++;
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    c->assembled = false;
+    return rc;
+}
+
+int oc_cloth_synchronize(oc_cloth c) {
+    if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
+    OC_HIP(hipStreamSynchronize(c->stream));
+    return OC_OK;
+}
+
+int oc_cloth_last_status(oc_cloth c, int32_t* iters, int32_t* status) {
+    if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
+    OC_HIP(hipMemcpyAsync(c->ctl_host, c->dc.ctl, sizeof(int) * 4, hipMemcpyDeviceToHost,
+                          c->stream));
+    OC_HIP(hipStreamSynchronize(c->stream));
+    if (iters) *iters = c->ctl_host[1];
+    if (status) *status = c->ctl_host[2];
+    return OC_OK;
+}
+
+int oc_cloth_num_pairs(oc_cloth c, int32_t* n) {
+    if (!c || !n) return fail(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    *n = c->topo.P;
+    return OC_OK;
+}
+
+int oc_cloth_get_pairs(oc_cloth c, int32_t* a, int32_t* b) {
+    if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
+    if (a) std::memcpy(a, c->topo.pair_a.data(), sizeof(int32_t) * c->topo.P);
+    if (b) std::memcpy(b, c->topo.pair_b.data(), sizeof(int32_t) * c->topo.P);
+    return OC_OK;
+}
+
+int oc_cloth_get_system(oc_cloth c, float* blocks, float* rhs) {
+    if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
+    const oc::GridTopology& t = c->topo;
+    const size_t N = t.N;
+    if (blocks) {
+        std::vector<float> planes(6 * N * std::max(t.n_planes, 1));
+        OC_HIP(hipMemcpyAsync(planes.data(), c->dc.blocks, planes.size() * sizeof(float),
+                              hipMemcpyDeviceToHost, c->stream));
+        OC_HIP(hipStreamSynchronize(c->stream));
+        for (int p = 0; p < t.P; p++)
+            for (int q = 0; q < 6; q++)
+                blocks[6 * (size_t)p + q] =
+                    planes[((size_t)t.pair_plane[p] * 6 + q) * N + (size_t)t.pair_a[p]];
+    }
+    if (rhs) return download_vec(c, c->dc.b, rhs);
+    return OC_OK;
+}
+
+}  // extern "C"

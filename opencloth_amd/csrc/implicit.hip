@@ -1,0 +1,447 @@
+// HIP kernels of the implicit mass-spring step (gfx950, wave64).
+//
+// One step (OpenCloth_ImplicitEuler/.../main.cpp:774-779 without the
+// post-step passes) is
+//   assemble  : per-vertex gather of spring forces and 3x3 Jacobian blocks,
+//               rhs b = h(f + h df/dx v), pair blocks C, CG start r = S(b - A dv)
+//   cg_spmv   : d = r + beta d on the tile and its halo in LDS, q = S A d,
+//               partial d.q                                    (x iterations)
+//   cg_axpy   : dv += alpha d, r -= alpha q, partial r.r        (x iterations)
+//   cg_finish : iteration count and status
+//   update    : v += dv, x += h v
+//
+// Layout: vertex vectors are SoA planes (x[c*N + i]).  Pair blocks live in
+// `planes`: the block of the pair (a, b = a + offset_p) is stored at
+// blocks[(p*6 + comp)*N + a], so neighbouring threads read neighbouring
+// addresses for every stencil direction.  The diagonal block
+// A_ii = m I - sum_j C_ij is never stored: (A d)_i = m d_i + sum_j C_ij (d_j - d_i).
+//
+// Arithmetic follows the reference's GLM expression order with no FMA
+// contraction (built with -ffp-contract=off), so per-spring forces and blocks
+// equal the CPU oracle bit for bit.  Reductions use one fixed tree: a 64-lane
+// shuffle tree per wave, the waves of a tile in order, then a lane-strided
+// pass over tile partials followed by the same lane tree.
+#include "oc_internal.hpp"
+
+namespace oc {
+
+namespace {
+
+constexpr int H = kHalo;
+
+struct V3 {
+    float x, y, z;
+};
+
+__device__ inline V3 ld3(const float* p, int N, int i) { return {p[i], p[N + i], p[2 * N + i]}; }
+__device__ inline void st3(float* p, int N, int i, V3 v) {
+    p[i] = v.x;
+    p[N + i] = v.y;
+    p[2 * N + i] = v.z;
+}
+__device__ inline float dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// symmetric (xx,yy,zz,xy,xz,yz) times vector, mat3*vec3 order
+__device__ inline V3 symmv(const float* c, V3 w) {
+    return {c[0] * w.x + c[3] * w.y + c[4] * w.z, c[3] * w.x + c[1] * w.y + c[5] * w.z,
+            c[4] * w.x + c[5] * w.y + c[2] * w.z};
+}
+
+// Force on p1 (IE:663-675), stiffness Jacobian K (IE:669-671) and
+// P = dp dp^T / l^2, in the reference's expression order.
+__device__ inline void spring_terms(V3 x1, V3 x2, V3 v1, V3 v2, float L0, float Ks, float Kd,
+                                    V3& f, float* K, float* P) {
+    V3 dp = {x1.x - x2.x, x1.y - x2.y, x1.z - x2.z};
+    V3 dv = {v1.x - v2.x, v1.y - v2.y, v1.z - v2.z};
+    float sq = dp.x * dp.x + dp.y * dp.y + dp.z * dp.z;
+    float dist = sqrtf(sq);
+    float dist2 = dist * dist;
+    float lo_l = L0 / dist;
+    P[0] = (dp.x * dp.x) / dist2;
+    P[1] = (dp.y * dp.y) / dist2;
+    P[2] = (dp.z * dp.z) / dist2;
+    P[3] = (dp.y * dp.x) / dist2;
+    P[4] = (dp.z * dp.x) / dist2;
+    P[5] = (dp.z * dp.y) / dist2;
+#pragma unroll
+    for (int c = 0; c < 3; c++) K[c] = Ks * (-1.0f + lo_l * (1.0f - P[c]));
+#pragma unroll
+    for (int c = 3; c < 6; c++) K[c] = Ks * (lo_l * (0.0f - P[c]));
+    float leftTerm = -Ks * (dist - L0);
+    float rightTerm = Kd * ((dv.x * dp.x + dv.y * dp.y + dv.z * dp.z) / dist);
+    float inv = 1.0f / sqrtf(sq);
+    float s = leftTerm + rightTerm;
+    f = {s * (dp.x * inv), s * (dp.y * inv), s * (dp.z * inv)};
+}
+
+// Force only (no Jacobian), same arithmetic as spring_terms.
+__device__ inline V3 spring_force(V3 x1, V3 x2, V3 v1, V3 v2, float L0, float Ks, float Kd) {
+    V3 dp = {x1.x - x2.x, x1.y - x2.y, x1.z - x2.z};
+    V3 dv = {v1.x - v2.x, v1.y - v2.y, v1.z - v2.z};
+    float sq = dp.x * dp.x + dp.y * dp.y + dp.z * dp.z;
+    float dist = sqrtf(sq);
+    float leftTerm = -Ks * (dist - L0);
+    float rightTerm = Kd * ((dv.x * dp.x + dv.y * dp.y + dv.z * dp.z) / dist);
+    float inv = 1.0f / sqrtf(sq);
+    float s = leftTerm + rightTerm;
+    return {s * (dp.x * inv), s * (dp.y * inv), s * (dp.z * inv)};
+}
+
+// Wave tree: lane l += lane l+off for off = 32..1; lane 0 ends with the sum.
+__device__ inline float wave_sum(float t) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) t += __shfl_down(t, off, 64);
+    return t;
+}
+
+// Tile partial: wave sums added in wave order.  Result valid in thread 0.
+template <int NT>
+__device__ inline float tile_sum(float t, float* ws) {
+    t = wave_sum(t);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) ws[w] = t;
+    __syncthreads();
+    float p = 0.0f;
+    if (threadIdx.x == 0) {
+        p = ws[0];
+#pragma unroll
+        for (int k = 1; k < NT / 64; k++) p = p + ws[k];
+    }
+    return p;
+}
+
+// Sum of nt tile partials: lane-strided then the wave tree; every block
+// computes the same value.  Returned to all threads.
+__device__ inline float partial_sum(const float* part, int nt, float* bc) {
+    if (threadIdx.x < 64) {
+        float s = 0.0f;
+        for (int k = threadIdx.x; k < nt; k += 64) s += part[k];
+        s = wave_sum(s);
+        if (threadIdx.x == 0) *bc = s;
+    }
+    __syncthreads();
+    return *bc;
+}
+
+// ---------------------------------------------------------------------------
+// forces (ComputeForces, IE:645-682): gather in spring order, pins get zero
+// ---------------------------------------------------------------------------
+template <int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_forces(DevCloth c, StepScalars s) {
+    const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+    const int bx = blockIdx.x % c.ntx, by = blockIdx.x / c.ntx;
+    const int gi = bx * TX + lx, gj = by * TY + ly;
+    if (gi >= c.nx || gj >= c.ny) return;
+    const int N = c.N, i = gj * c.nx + gi;
+    V3 acc = {0.0f, 0.0f, 0.0f};
+    if (!c.pin[i]) {
+        V3 xi = ld3(c.x, N, i), vi = ld3(c.v, N, i);
+        acc = {0.0f + s.mass * s.gx, 0.0f + s.mass * s.gy, 0.0f + s.mass * s.gz};
+        for (int k = 0; k < c.wa; k++) {
+            const int sp = c.ainc_s[k * N + i];
+            if (sp < 0) break;
+            const uint16_t m = c.ainc_m[k * N + i];
+            const int j = i + slot_dj(m) * c.nx + slot_di(m);
+            V3 xj = ld3(c.x, N, j), vj = ld3(c.v, N, j);
+            const bool second = (m >> 8) & 1u;
+            V3 f = second ? spring_force(xj, xi, vj, vi, c.L0[sp], c.ks[sp], c.kd[sp])
+                          : spring_force(xi, xj, vi, vj, c.L0[sp], c.ks[sp], c.kd[sp]);
+            if (second)
+                acc = {acc.x - f.x, acc.y - f.y, acc.z - f.z};
+            else
+                acc = {acc.x + f.x, acc.y + f.y, acc.z + f.z};
+        }
+    }
+    st3(c.f, N, i, acc);
+}
+
+// ---------------------------------------------------------------------------
+// assembly + CG start
+// ---------------------------------------------------------------------------
+template <int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) {
+    constexpr int NT = TX * TY;
+    __shared__ float ws[NT / 64];
+    const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+    const int bx = blockIdx.x % c.ntx, by = blockIdx.x / c.ntx;
+    const int gi = bx * TX + lx, gj = by * TY + ly;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c.ctl[0] = 0;
+        c.ctl[1] = 0;
+        c.ctl[2] = 0;
+    }
+    float t = 0.0f;
+    if (gi < c.nx && gj < c.ny) {
+        const int N = c.N, i = gj * c.nx + gi;
+        const float h = s.h, hh = s.hh;
+        const bool pinned = c.pin[i] != 0;
+        V3 xi = ld3(c.x, N, i), vi = ld3(c.v, N, i);
+        V3 dvi = s.warm ? ld3(c.dv, N, i) : V3{0.0f, 0.0f, 0.0f};
+        if (!s.warm) st3(c.dv, N, i, dvi);
+        V3 acc = {0.0f, 0.0f, 0.0f}, kv = {0.0f, 0.0f, 0.0f};
+        if (!pinned) acc = {0.0f + s.mass * s.gx, 0.0f + s.mass * s.gy, 0.0f + s.mass * s.gz};
+        V3 adv = {s.mass * dvi.x, s.mass * dvi.y, s.mass * dvi.z};
+        float C[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        for (int k = 0; k < c.wa; k++) {
+            const int sp = c.ainc_s[k * N + i];
+            if (sp < 0) break;
+            const uint16_t m = c.ainc_m[k * N + i];
+            const int j = i + slot_dj(m) * c.nx + slot_di(m);
+            V3 xj = ld3(c.x, N, j), vj = ld3(c.v, N, j);
+            const bool second = (m >> 8) & 1u;
+            V3 f;
+            float K[6], P[6];
+            const float kd = c.kd[sp];
+            if (second)
+                spring_terms(xj, xi, vj, vi, c.L0[sp], c.ks[sp], kd, f, K, P);
+            else
+                spring_terms(xi, xj, vi, vj, c.L0[sp], c.ks[sp], kd, f, K, P);
+#pragma unroll
+            for (int q = 0; q < 6; q++) C[q] += h * (kd * P[q]) + hh * K[q];
+            V3 w = second ? V3{vj.x - vi.x, vj.y - vi.y, vj.z - vi.z}
+                          : V3{vi.x - vj.x, vi.y - vj.y, vi.z - vj.z};
+            V3 kw = symmv(K, w);
+            if (second) {
+                acc = {acc.x - f.x, acc.y - f.y, acc.z - f.z};
+                kv = {kv.x - kw.x, kv.y - kw.y, kv.z - kw.z};
+            } else {
+                acc = {acc.x + f.x, acc.y + f.y, acc.z + f.z};
+                kv = {kv.x + kw.x, kv.y + kw.y, kv.z + kw.z};
+            }
+            if ((m >> 13) & 1u) {  // last spring of this pair
+                if ((m >> 14) & 1u) {
+                    const int plane = (m >> 9) & 15;
+#pragma unroll
+                    for (int q = 0; q < 6; q++) c.blocks[(size_t)(plane * 6 + q) * N + i] = C[q];
+                }
+                V3 dvj = s.warm ? ld3(c.dv, N, j) : V3{0.0f, 0.0f, 0.0f};
+                V3 wd = {dvj.x - dvi.x, dvj.y - dvi.y, dvj.z - dvi.z};
+                V3 a = symmv(C, wd);
+                adv = {adv.x + a.x, adv.y + a.y, adv.z + a.z};
+#pragma unroll
+                for (int q = 0; q < 6; q++) C[q] = 0.0f;
+            }
+        }
+        V3 b;
+        b.x = pinned ? 0.0f : h * (acc.x + h * kv.x);
+        b.y = pinned ? 0.0f : h * (acc.y + h * kv.y);
+        b.z = pinned ? 0.0f : h * (acc.z + h * kv.z);
+        st3(c.b, N, i, b);
+        V3 r = pinned ? V3{0.0f, 0.0f, 0.0f} : V3{b.x - adv.x, b.y - adv.y, b.z - adv.z};
+        st3(c.r, N, i, r);
+        st3(c.d0, N, i, r);
+        t = dot3(r, r);
+    }
+    float p = tile_sum<NT>(t, ws);
+    if (threadIdx.x == 0) c.del_part[blockIdx.x] = p;
+}
+
+// ---------------------------------------------------------------------------
+// CG iteration k, part 1: direction update on the tile + halo, q = S A d
+// ---------------------------------------------------------------------------
+template <int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_cg_spmv(DevCloth c, StepScalars s, int k) {
+    constexpr int NT = TX * TY;
+    constexpr int WX = TX + 2 * H, WY = TY + 2 * H;
+    __shared__ float wd[3][WY][WX];
+    __shared__ float ws[NT / 64];
+    __shared__ float bc;
+    if (c.ctl[0]) return;
+    const int nt = c.nt, N = c.N;
+    const float delta = partial_sum(c.del_part + (size_t)k * nt, nt, &bc);
+    const float thresh = s.tol_mode == 1 ? s.tol2 * (k == 0 ? delta : c.scal[0]) : s.tol2;
+    const bool stop = s.tol_mode != 2 && !(delta > thresh);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c.scal[k] = delta;
+        if (stop) {
+            c.ctl[0] = 1;
+            c.ctl[1] = k;
+        }
+    }
+    if (stop) return;
+    // beta = 0 once r.r has underflowed to 0 (fixed iteration counts)
+    const float delta_old = k > 0 ? c.scal[k - 1] : 0.0f;
+    const float beta = (k > 0 && delta_old != 0.0f) ? delta / delta_old : 0.0f;
+    const float* dprev = (k & 1) ? c.d0 : c.d1;
+    float* dcur = (k & 1) ? c.d1 : c.d0;
+    const int bx = blockIdx.x % c.ntx, by = blockIdx.x / c.ntx;
+    const int x0 = bx * TX - H, y0 = by * TY - H;
+    for (int idx = threadIdx.x; idx < WX * WY; idx += NT) {
+        const int wy = idx / WX, wx = idx % WX;
+        const int gi = x0 + wx, gj = y0 + wy;
+        V3 d = {0.0f, 0.0f, 0.0f};
+        if (gi >= 0 && gi < c.nx && gj >= 0 && gj < c.ny) {
+            const int i = gj * c.nx + gi;
+            if (k == 0) {
+                d = ld3(c.d0, N, i);
+            } else {
+                V3 r = ld3(c.r, N, i), dp = ld3(dprev, N, i);
+                d = {r.x + beta * dp.x, r.y + beta * dp.y, r.z + beta * dp.z};
+                if (wx >= H && wx < H + TX && wy >= H && wy < H + TY) st3(dcur, N, i, d);
+            }
+        }
+        wd[0][wy][wx] = d.x;
+        wd[1][wy][wx] = d.y;
+        wd[2][wy][wx] = d.z;
+    }
+    __syncthreads();
+    const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+    const int gi = bx * TX + lx, gj = by * TY + ly;
+    float t = 0.0f;
+    if (gi < c.nx && gj < c.ny) {
+        const int i = gj * c.nx + gi;
+        const V3 di = {wd[0][ly + H][lx + H], wd[1][ly + H][lx + H], wd[2][ly + H][lx + H]};
+        V3 acc = {s.mass * di.x, s.mass * di.y, s.mass * di.z};
+        for (int sl = 0; sl < c.wp; sl++) {
+            const uint16_t m = c.pair_m[sl * N + i];
+            if (m == kEmptySlot) break;
+            const int ddi = slot_di(m), ddj = slot_dj(m);
+            const int plane = (m >> 8) & 15;
+            const int owner = (m >> 12) & 1;
+            const int j = i + ddj * c.nx + ddi;
+            const int blk = owner ? i : j;
+            const float* bp = c.blocks + (size_t)plane * 6 * N + blk;
+            float C[6];
+#pragma unroll
+            for (int q = 0; q < 6; q++) C[q] = bp[(size_t)q * N];
+            const int wy = ly + H + ddj, wx = lx + H + ddi;
+            V3 w = {wd[0][wy][wx] - di.x, wd[1][wy][wx] - di.y, wd[2][wy][wx] - di.z};
+            V3 a = symmv(C, w);
+            acc = {acc.x + a.x, acc.y + a.y, acc.z + a.z};
+        }
+        V3 q = c.pin[i] ? V3{0.0f, 0.0f, 0.0f} : acc;
+        st3(c.q, N, i, q);
+        t = dot3(di, q);
+    }
+    float p = tile_sum<NT>(t, ws);
+    if (threadIdx.x == 0) c.sig_part[(size_t)k * nt + blockIdx.x] = p;
+}
+
+// CG iteration k, part 2: dv += alpha d, r -= alpha q, partial r.r
+template <int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_cg_axpy(DevCloth c, StepScalars s, int k) {
+    constexpr int NT = TX * TY;
+    __shared__ float ws[NT / 64];
+    __shared__ float bc;
+    if (c.ctl[0]) return;
+    const int nt = c.nt, N = c.N;
+    const float sigma = partial_sum(c.sig_part + (size_t)k * nt, nt, &bc);
+    const float delta = c.scal[k];
+    if (sigma < 0.0f || !isfinite(sigma)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            c.ctl[0] = 1;
+            c.ctl[1] = k;
+            c.ctl[2] |= OC_SOLVE_BREAKDOWN;
+        }
+        return;
+    }
+    // sigma == 0 only after r (hence d) underflowed: no update, no 0/0
+    const float alpha = sigma == 0.0f ? 0.0f : delta / sigma;
+    const float* dcur = (k & 1) ? c.d1 : c.d0;
+    const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+    const int bx = blockIdx.x % c.ntx, by = blockIdx.x / c.ntx;
+    const int gi = bx * TX + lx, gj = by * TY + ly;
+    float t = 0.0f;
+    if (gi < c.nx && gj < c.ny) {
+        const int i = gj * c.nx + gi;
+        V3 d = ld3(dcur, N, i), q = ld3(c.q, N, i), x = ld3(c.dv, N, i), r = ld3(c.r, N, i);
+        x = {x.x + alpha * d.x, x.y + alpha * d.y, x.z + alpha * d.z};
+        r = {r.x - alpha * q.x, r.y - alpha * q.y, r.z - alpha * q.z};
+        st3(c.dv, N, i, x);
+        st3(c.r, N, i, r);
+        t = dot3(r, r);
+    }
+    float p = tile_sum<NT>(t, ws);
+    if (threadIdx.x == 0) c.del_part[(size_t)(k + 1) * nt + blockIdx.x] = p;
+}
+
+// Iteration count and status of the solve (one 64-thread block).
+__global__ void k_cg_finish(DevCloth c, StepScalars s) {
+    __shared__ float bc;
+    const int done = c.ctl[0];
+    int it;
+    float delta;
+    if (done) {
+        it = c.ctl[1];
+        delta = c.scal[it];
+    } else {
+        it = s.max_it;
+        delta = partial_sum(c.del_part + (size_t)it * c.nt, c.nt, &bc);
+    }
+    // r0.r0: written by the first spmv launch, or equal to delta without iterations
+    const float delta0 = s.max_it > 0 ? c.scal[0] : delta;
+    if (threadIdx.x == 0) {
+        const float thresh = s.tol_mode == 1 ? s.tol2 * delta0 : s.tol2;
+        int st = c.ctl[2];
+        if (s.tol_mode != 2 && delta > thresh && !(st & OC_SOLVE_BREAKDOWN))
+            st |= OC_SOLVE_NOT_CONVERGED;
+        if (!isfinite(delta)) st |= OC_SOLVE_NONFINITE;
+        c.ctl[1] = it;
+        c.ctl[2] = st;
+        c.ctl[3] |= st;
+    }
+}
+
+// v += dv, x += h v, optional ground clamp (IE:693-699).
+__global__ __launch_bounds__(256) void k_update(DevCloth c, StepScalars s) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= c.N) return;
+    const int N = c.N;
+    V3 v = ld3(c.v, N, i), dv = ld3(c.dv, N, i), x = ld3(c.x, N, i);
+    v = {v.x + dv.x, v.y + dv.y, v.z + dv.z};
+    x = {x.x + s.h * v.x, x.y + s.h * v.y, x.z + s.h * v.z};
+    if ((s.flags & OC_FLAG_GROUND_CLAMP) && x.y < 0.0f) x.y = 0.0f;
+    st3(c.v, N, i, v);
+    st3(c.x, N, i, x);
+    if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) && isfinite(v.y) &&
+          isfinite(v.z))) {
+        atomicOr(c.ctl + 2, OC_SOLVE_NONFINITE);
+        atomicOr(c.ctl + 3, OC_SOLVE_NONFINITE);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_forces(const DevCloth& c, const StepScalars& s, hipStream_t st) {
+    hipLaunchKernelGGL((k_forces<kTileX, kTileY>), dim3(c.nt), dim3(kTileX * kTileY), 0, st, c, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_assemble(const DevCloth& c, const StepScalars& s, hipStream_t st) {
+    hipLaunchKernelGGL((k_assemble<kTileX, kTileY>), dim3(c.nt), dim3(kTileX * kTileY), 0, st, c,
+                       s);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_spmv(const DevCloth& c, const StepScalars& s, int k, hipStream_t st) {
+    hipLaunchKernelGGL((k_cg_spmv<kTileX, kTileY>), dim3(c.nt), dim3(kTileX * kTileY), 0, st, c,
+                       s, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_axpy(const DevCloth& c, const StepScalars& s, int k, hipStream_t st) {
+    hipLaunchKernelGGL((k_cg_axpy<kTileX, kTileY>), dim3(c.nt), dim3(kTileX * kTileY), 0, st, c,
+                       s, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_finish(const DevCloth& c, const StepScalars& s, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), dim3(64), 0, st, c, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg(const DevCloth& c, const StepScalars& s, hipStream_t st) {
+    hipError_t e;
+    for (int k = 0; k < s.max_it; k++) {
+        if ((e = launch_cg_spmv(c, s, k, st)) != hipSuccess) return e;
+        if ((e = launch_cg_axpy(c, s, k, st)) != hipSuccess) return e;
+    }
+    return launch_cg_finish(c, s, st);
+}
+
+hipError_t launch_update(const DevCloth& c, const StepScalars& s, hipStream_t st) {
+    hipLaunchKernelGGL(k_update, dim3((c.N + 255) / 256), dim3(256), 0, st, c, s);
+    return hipGetLastError();
+}
+
+}  // namespace oc
+

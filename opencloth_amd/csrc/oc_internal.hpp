@@ -1,0 +1,97 @@
+// Internal declarations shared by the host C ABI and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "opencloth.h"
+
+namespace oc {
+
+// A spring may join vertices at most kHalo rows and kHalo columns apart
+// (the reference's bend springs span 2, IE:519-530).  The SpMV stages a
+// kHalo-wide halo around each tile in LDS.
+constexpr int kHalo = 2;
+constexpr int kMaxPlanes = 16;
+constexpr int kMaxAssemblyWidth = 32;
+constexpr int kMaxPairWidth = 24;
+constexpr uint16_t kEmptySlot = 0xFFFFu;
+
+// ---- per-vertex slot encodings (16 bits) ----------------------------------
+// assembly slot (one incident spring):
+//   bits 0-3 di+8, 4-7 dj+8 of the other end relative to this vertex,
+//   bit 8 this vertex is the spring's p2, bits 9-12 plane of the pair,
+//   bit 13 last spring of its pair in this row, bit 14 this vertex owns the
+//   pair block (it is the pair's lower index).
+// pair slot (one incident pair, SpMV):
+//   bits 0-3 di+8, 4-7 dj+8, bits 8-11 plane, bit 12 this vertex owns it.
+__host__ __device__ inline int slot_di(uint16_t m) { return int(m & 15u) - 8; }
+__host__ __device__ inline int slot_dj(uint16_t m) { return int((m >> 4) & 15u) - 8; }
+
+// Grid topology in the device layout, built on the host from a spring list.
+struct GridTopology {
+    int nx = 0, ny = 0, N = 0, S = 0, P = 0;
+    int n_planes = 0;
+    int di[kMaxPlanes] = {}, dj[kMaxPlanes] = {};  // offset of the plane's upper vertex
+    int wa = 0;                                    // assembly slots per vertex
+    int wp = 0;                                    // pair slots per vertex
+    std::vector<int32_t> ainc_spring;              // wa*N, -1 = empty (slot-major)
+    std::vector<uint16_t> ainc_meta;               // wa*N
+    std::vector<uint16_t> pair_meta;               // wp*N, kEmptySlot = empty
+    std::vector<int32_t> pair_a, pair_b, pair_plane;
+    std::vector<int32_t> spring_pair;
+};
+
+// Builds the tables; returns OC_OK or an error code with a message in *err.
+int build_grid_topology(int nx, int ny, const oc_spring* springs, int n_springs,
+                        GridTopology* out, std::string* err);
+
+// ---- device views ---------------------------------------------------------
+// SoA vectors: component c of vertex i at ptr[c*N + i].
+struct DevCloth {
+    int nx, ny, N, ntx, nty, nt;
+    int wa, wp;
+    float* x;
+    float* v;
+    float* dv;
+    float* r;
+    float* d0;
+    float* d1;
+    float* q;
+    float* b;
+    float* f;
+    const uint8_t* pin;
+    float* blocks;  // n_planes*6*N
+    const int32_t* ainc_s;
+    const uint16_t* ainc_m;
+    const uint16_t* pair_m;
+    const float* L0;
+    const float* ks;
+    const float* kd;
+    float* del_part;  // (max_it+1)*nt
+    float* sig_part;  // max_it*nt
+    float* scal;      // max_it+1 reduced r.r values
+    int* ctl;         // [0] done, [1] iterations, [2] status bits
+};
+
+struct StepScalars {
+    float mass, gx, gy, gz, h, hh, tol2;
+    int max_it, tol_mode, warm, flags;
+};
+
+// Kernel launchers (implicit.hip).  All return hipError_t of the launch.
+hipError_t launch_forces(const DevCloth& c, const StepScalars& s, hipStream_t st);
+hipError_t launch_assemble(const DevCloth& c, const StepScalars& s, hipStream_t st);
+hipError_t launch_cg(const DevCloth& c, const StepScalars& s, hipStream_t st);
+hipError_t launch_cg_spmv(const DevCloth& c, const StepScalars& s, int k, hipStream_t st);
+hipError_t launch_cg_axpy(const DevCloth& c, const StepScalars& s, int k, hipStream_t st);
+hipError_t launch_cg_finish(const DevCloth& c, const StepScalars& s, hipStream_t st);
+hipError_t launch_update(const DevCloth& c, const StepScalars& s, hipStream_t st);
+// Tile shape of the implicit kernels.
+constexpr int kTileX = 32;
+constexpr int kTileY = 8;
+
+}  // namespace oc

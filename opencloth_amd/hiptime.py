@@ -1,0 +1,61 @@
+"""Minimal ctypes access to HIP events for timing work on a given stream
+(the library's handles launch on the stream set with oc_cloth_set_stream;
+the default is the null stream)."""
+from __future__ import annotations
+
+import ctypes as C
+
+_hip = None
+
+
+def hip() -> C.CDLL:
+    global _hip
+    if _hip is None:
+        for p in ("/opt/rocm/lib/libamdhip64.so", "libamdhip64.so"):
+            try:
+                _hip = C.CDLL(p)
+                break
+            except OSError:
+                continue
+        if _hip is None:
+            raise RuntimeError("libamdhip64.so not found")
+        _hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        _hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        _hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        _hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        _hip.hipEventDestroy.argtypes = [C.c_void_p]
+        _hip.hipStreamSynchronize.argtypes = [C.c_void_p]
+    return _hip
+
+
+class Event:
+    def __init__(self):
+        self.e = C.c_void_p()
+        rc = hip().hipEventCreate(C.byref(self.e))
+        if rc != 0:
+            raise RuntimeError(f"hipEventCreate failed ({rc})")
+
+    def record(self, stream=None):
+        rc = hip().hipEventRecord(self.e, C.c_void_p(stream or 0))
+        if rc != 0:
+            raise RuntimeError(f"hipEventRecord failed ({rc})")
+
+    def synchronize(self):
+        hip().hipEventSynchronize(self.e)
+
+    def elapsed_ms(self, end: "Event") -> float:
+        t = C.c_float()
+        rc = hip().hipEventElapsedTime(C.byref(t), self.e, end.e)
+        if rc != 0:
+            raise RuntimeError(f"hipEventElapsedTime failed ({rc})")
+        return t.value
+
+    def __del__(self):
+        try:
+            hip().hipEventDestroy(self.e)
+        except Exception:
+            pass
+
+
+def device_synchronize():
+    hip().hipDeviceSynchronize()

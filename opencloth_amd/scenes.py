@@ -1,0 +1,204 @@
+"""Cloth scenes: grid positions, the reference spring list, pins and the
+workload configurations, as numpy arrays ready for the C ABI.
+
+Everything here is host-side input generation.  The arithmetic of the grid
+and the rest lengths is float32 in the reference's expression order so the
+arrays equal what the reference's InitGL builds:
+
+* positions  -- OpenCloth_ImplicitEuler/.../main.cpp:462-466 (IE)
+* springs    -- IE:496-530 with AddSpring IE:336-346
+* constants  -- IE:309-316 (the reference stores the damping Kd negative)
+
+Random inputs use SplitMix64 with Box-Muller, so any host can regenerate the
+same scene from a seed.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# Spring record in the reference layout (IE:68-73): 24 bytes.
+SPRING_DTYPE = np.dtype(
+    [("p1", "<i4"), ("p2", "<i4"), ("rest_length", "<f4"), ("Ks", "<f4"), ("Kd", "<f4"), ("type", "<i4")]
+)
+STRUCTURAL, SHEAR, BEND = 0, 1, 2
+
+
+# ---------------------------------------------------------------------------
+# random numbers
+# ---------------------------------------------------------------------------
+_M64 = (1 << 64) - 1
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    """n outputs of SplitMix64 started at `seed` (uint64)."""
+    with np.errstate(over="ignore"):
+        k = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed & _M64) + k * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed: int, n: int) -> np.ndarray:
+    """n doubles in [0, 1) from SplitMix64 (top 53 bits)."""
+    return (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+def normal(seed: int, n: int) -> np.ndarray:
+    """n standard normals by Box-Muller from uniform(seed, 2*ceil(n/2))."""
+    m = (n + 1) // 2
+    u = uniform(seed, 2 * m)
+    u1 = 1.0 - u[:m]  # (0, 1]
+    u2 = u[m:]
+    r = np.sqrt(-2.0 * np.log(u1))
+    z = np.concatenate([r * np.cos(2 * math.pi * u2), r * np.sin(2 * math.pi * u2)])
+    return z[:n]
+
+
+# ---------------------------------------------------------------------------
+# grid and springs
+# ---------------------------------------------------------------------------
+def grid_positions(nx: int, ny: int, fullsize: float = 4.0, y0: float = 0.0) -> np.ndarray:
+    """(N,3) float32; vertex k = j*nx + i at
+    (((i/(nx-1))*2-1)*halfsize, y0, (j/(ny-1))*fullsize)   (IE:462-466,
+    which uses y0 = fullsize+1)."""
+    f32 = np.float32
+    fs = f32(fullsize)
+    half = fs / f32(2.0)
+    i = np.arange(nx, dtype=np.float32)
+    j = np.arange(ny, dtype=np.float32)
+    xi = ((i / f32(nx - 1)) * f32(2) - f32(1)) * half
+    zj = (j / f32(ny - 1)) * fs
+    X = np.empty((ny, nx, 3), np.float32)
+    X[:, :, 0] = xi[None, :]
+    X[:, :, 1] = f32(y0)
+    X[:, :, 2] = zj[:, None]
+    return X.reshape(-1, 3)
+
+
+def grid_pairs(nx: int, ny: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """(p1, p2, type) of the reference spring list (IE:496-530), in order:
+    horizontal, vertical, two shears per quad, bend rows (each row's last bend
+    twice), bend columns (each column's last bend twice)."""
+    u, v = nx, ny
+    if u < 3 or v < 3:
+        raise ValueError("the reference spring topology needs at least 3x3 vertices")
+    ps, qs, ts = [], [], []
+    # horizontal: for l1 < v, l2 < u-1
+    l1, l2 = np.meshgrid(np.arange(v), np.arange(u - 1), indexing="ij")
+    ps.append((l1 * u + l2).ravel()); qs.append((l1 * u + l2 + 1).ravel()); ts.append(STRUCTURAL)
+    # vertical: for l1 < u, l2 < v-1
+    l1, l2 = np.meshgrid(np.arange(u), np.arange(v - 1), indexing="ij")
+    ps.append((l2 * u + l1).ravel()); qs.append(((l2 + 1) * u + l1).ravel()); ts.append(STRUCTURAL)
+    # shear: two per quad, interleaved
+    l1, l2 = np.meshgrid(np.arange(v - 1), np.arange(u - 1), indexing="ij")
+    a1 = (l1 * u + l2).ravel(); b1 = ((l1 + 1) * u + l2 + 1).ravel()
+    a2 = ((l1 + 1) * u + l2).ravel(); b2 = (l1 * u + l2 + 1).ravel()
+    ps.append(np.stack([a1, a2], 1).ravel()); qs.append(np.stack([b1, b2], 1).ravel()); ts.append(SHEAR)
+    # bend rows: per row u-2 springs then the duplicate of the last
+    l1, l2 = np.meshgrid(np.arange(v), np.arange(u - 2), indexing="ij")
+    a = l1 * u + l2; b = l1 * u + l2 + 2
+    a = np.concatenate([a, (np.arange(v) * u + (u - 3))[:, None]], 1)
+    b = np.concatenate([b, (np.arange(v) * u + (u - 1))[:, None]], 1)
+    ps.append(a.ravel()); qs.append(b.ravel()); ts.append(BEND)
+    # bend columns
+    l1, l2 = np.meshgrid(np.arange(u), np.arange(v - 2), indexing="ij")
+    a = l2 * u + l1; b = (l2 + 2) * u + l1
+    a = np.concatenate([a, ((v - 3) * u + np.arange(u))[:, None]], 1)
+    b = np.concatenate([b, ((v - 1) * u + np.arange(u))[:, None]], 1)
+    ps.append(a.ravel()); qs.append(b.ravel()); ts.append(BEND)
+    p1 = np.concatenate(ps).astype(np.int32)
+    p2 = np.concatenate(qs).astype(np.int32)
+    ty = np.concatenate([np.full(len(x), t, np.int32) for x, t in zip(ps, ts)])
+    return p1, p2, ty
+
+
+def rest_lengths(X: np.ndarray, p1: np.ndarray, p2: np.ndarray) -> np.ndarray:
+    """sqrt(dot(dp,dp)) in float32, dp = X[p1]-X[p2] (IE:343-344)."""
+    d = X[p1].astype(np.float32) - X[p2].astype(np.float32)
+    s = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]
+    s = s + d[:, 2] * d[:, 2]
+    return np.sqrt(s.astype(np.float32))
+
+
+def grid_springs(nx, ny, X, ks=(50.0, 50.0, 50.0), kd=(-0.25, -0.25, -0.25)) -> np.ndarray:
+    """Reference spring list as a SPRING_DTYPE array.  ks/kd per type
+    (structural, shear, bend); kd uses the reference sign (negative damps)."""
+    p1, p2, ty = grid_pairs(nx, ny)
+    s = np.zeros(len(p1), SPRING_DTYPE)
+    s["p1"], s["p2"], s["type"] = p1, p2, ty
+    s["rest_length"] = rest_lengths(X, p1, p2)
+    s["Ks"] = np.asarray(ks, np.float32)[ty]
+    s["Kd"] = np.asarray(kd, np.float32)[ty]
+    return s
+
+
+# ---------------------------------------------------------------------------
+# step parameters and configurations
+# ---------------------------------------------------------------------------
+TOL_ABSOLUTE, TOL_RELATIVE, TOL_NONE = 0, 1, 2
+
+
+@dataclass
+class StepParams:
+    mass: float = 1.0
+    gravity: tuple = (0.0, -9.81, 0.0)
+    dt: float = 1.0 / 60.0
+    cg_max_iters: int = 10
+    cg_tol: float = 1e-3
+    cg_tol_mode: int = TOL_ABSOLUTE
+    warm_start: bool = True
+    ground_clamp: bool = False
+
+
+@dataclass
+class Scene:
+    name: str
+    nx: int
+    ny: int
+    X: np.ndarray  # (N,3) float32
+    V: np.ndarray  # (N,3) float32
+    springs: np.ndarray  # SPRING_DTYPE
+    pinned: np.ndarray  # (N,) uint8
+    params: StepParams
+    steps: int
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def n_vertices(self) -> int:
+        return self.nx * self.ny
+
+
+def perturb(X: np.ndarray, sigma: float, seed: int) -> np.ndarray:
+    """X + N(0, sigma) per coordinate, float32."""
+    z = normal(seed, X.size).reshape(X.shape)
+    return (X.astype(np.float64) + sigma * z).astype(np.float32)
+
+
+def plain_scene(n: int = 21, steps: int = 300) -> Scene:
+    """21x21 over a 4x4 square at y=0, ks 50, kd 0.25, mass 1, g -9.81,
+    dt 1/60, two top corners pinned, CG 10 iterations with absolute tol 1e-3,
+    300 steps from rest."""
+    X = grid_positions(n, n, 4.0, 0.0)
+    springs = grid_springs(n, n, X)
+    pinned = np.zeros(n * n, np.uint8)
+    pinned[0] = pinned[n - 1] = 1
+    p = StepParams(dt=1.0 / 60.0, cg_max_iters=10, cg_tol=1e-3, cg_tol_mode=TOL_ABSOLUTE)
+    return Scene(f"plain_{n}x{n}", n, n, X, np.zeros_like(X), springs, pinned, p, steps)
+
+
+def large_scene(n: int = 512, steps: int = 100, seed: int = 0) -> Scene:
+    """512x512 over the 4x4 square, positions perturbed by N(0, 1e-3*spacing)
+    from `seed` (rest lengths from the unperturbed grid), top edge (row 0)
+    pinned, dt 1/120, CG fixed at 30 iterations."""
+    X0 = grid_positions(n, n, 4.0, 0.0)
+    springs = grid_springs(n, n, X0)
+    spacing = 4.0 / (n - 1)
+    X = perturb(X0, 1e-3 * spacing, seed)
+    pinned = np.zeros(n * n, np.uint8)
+    pinned[:n] = 1
+    p = StepParams(dt=1.0 / 120.0, cg_max_iters=30, cg_tol=0.0, cg_tol_mode=TOL_NONE)
+    return Scene(f"large_{n}x{n}", n, n, X, np.zeros_like(X), springs, pinned, p, steps)
