@@ -11,7 +11,7 @@
 //   update    : v += dv, x += h v
 //
 // Layout: vertex vectors are SoA planes (x[c*N + i]).  Pair blocks live in
-// `planes`: the block of the pair (a, b = a + offset_p) is stored at
+// planes: the block of the pair (a, b = a + offset_p) is stored at
 // blocks[(p*6 + comp)*N + a], so neighbouring threads read neighbouring
 // addresses for every stencil direction.  The diagonal block
 // A_ii = m I - sum_j C_ij is never stored: (A d)_i = m d_i + sum_j C_ij (d_j - d_i).
@@ -20,7 +20,12 @@
 // contraction (built with -ffp-contract=off), so per-spring forces and blocks
 // equal the CPU oracle bit for bit.  Reductions use one fixed tree: a 64-lane
 // shuffle tree per wave, the waves of a tile in order, then a lane-strided
-// pass over tile partials followed by the same lane tree.
+// pass over tile partials followed by the same lane tree.  Partials are
+// indexed by tile, so the block-to-tile mapping does not change any sum.
+//
+// Every CG kernel issues all loads that do not depend on the previous
+// reduction (its own vectors, pair blocks, the partials) before waiting, so
+// the cross-tile reduction overlaps the memory traffic.
 #include "oc_internal.hpp"
 
 namespace oc {
@@ -28,6 +33,7 @@ namespace oc {
 namespace {
 
 constexpr int H = kHalo;
+constexpr int kPartialRegs = 16;  // partials per lane loaded up front (1024 tiles)
 
 struct V3 {
     float x, y, z;
@@ -109,17 +115,40 @@ __device__ inline float tile_sum(float t, float* ws) {
     return p;
 }
 
-// Sum of nt tile partials: lane-strided then the wave tree; every block
-// computes the same value.  Returned to all threads.
+// Cross-tile sum in two halves so the loads can be issued early.  Lane l of
+// wave 0 sums tiles l, l+64, l+128, ... in order, then the wave tree.
+__device__ inline void partial_load(const float* part, int nt, float* v) {
+#pragma unroll
+    for (int j = 0; j < kPartialRegs; j++) {
+        const int k = threadIdx.x + 64 * j;
+        v[j] = k < nt ? part[k] : 0.0f;
+    }
+}
+__device__ inline float partial_finish(const float* part, int nt, const float* v) {
+    float s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < kPartialRegs; j++) s += v[j];
+    for (int k = threadIdx.x + 64 * kPartialRegs; k < nt; k += 64) s += part[k];
+    return wave_sum(s);
+}
+
+// Sum of nt tile partials, returned to all threads (one-shot form).
 __device__ inline float partial_sum(const float* part, int nt, float* bc) {
     if (threadIdx.x < 64) {
-        float s = 0.0f;
-        for (int k = threadIdx.x; k < nt; k += 64) s += part[k];
-        s = wave_sum(s);
+        float v[kPartialRegs];
+        partial_load(part, nt, v);
+        const float s = partial_finish(part, nt, v);
         if (threadIdx.x == 0) *bc = s;
     }
     __syncthreads();
     return *bc;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; each XCD gets a
+// contiguous band of tiles so that halo and shared-block re-reads hit its L2.
+__device__ inline int tile_of_block(int b, int nt) {
+    if ((nt & 7) == 0) return (b & 7) * (nt >> 3) + (b >> 3);
+    return b;
 }
 
 // ---------------------------------------------------------------------------
@@ -128,7 +157,8 @@ __device__ inline float partial_sum(const float* part, int nt, float* bc) {
 template <int TX, int TY>
 __global__ __launch_bounds__(TX* TY) void k_forces(DevCloth c, StepScalars s) {
     const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
-    const int bx = blockIdx.x % c.ntx, by = blockIdx.x / c.ntx;
+    const int t = tile_of_block(blockIdx.x, c.nt);
+    const int bx = t % c.ntx, by = t / c.ntx;
     const int gi = bx * TX + lx, gj = by * TY + ly;
     if (gi >= c.nx || gj >= c.ny) return;
     const int N = c.N, i = gj * c.nx + gi;
@@ -157,12 +187,13 @@ __global__ __launch_bounds__(TX* TY) void k_forces(DevCloth c, StepScalars s) {
 // ---------------------------------------------------------------------------
 // assembly + CG start
 // ---------------------------------------------------------------------------
-template <int TX, int TY>
+template <int TX, int TY, int WA>
 __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) {
     constexpr int NT = TX * TY;
     __shared__ float ws[NT / 64];
     const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
-    const int bx = blockIdx.x % c.ntx, by = blockIdx.x / c.ntx;
+    const int tile = tile_of_block(blockIdx.x, c.nt);
+    const int bx = tile % c.ntx, by = tile / c.ntx;
     const int gi = bx * TX + lx, gj = by * TY + ly;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         c.ctl[0] = 0;
@@ -173,6 +204,13 @@ __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) 
     if (gi < c.nx && gj < c.ny) {
         const int N = c.N, i = gj * c.nx + gi;
         const float h = s.h, hh = s.hh;
+        int sps[WA];
+        uint16_t ms[WA];
+#pragma unroll
+        for (int k = 0; k < WA; k++) {
+            sps[k] = k < c.wa ? c.ainc_s[k * N + i] : -1;
+            ms[k] = k < c.wa ? c.ainc_m[k * N + i] : kEmptySlot;
+        }
         const bool pinned = c.pin[i] != 0;
         V3 xi = ld3(c.x, N, i), vi = ld3(c.v, N, i);
         V3 dvi = s.warm ? ld3(c.dv, N, i) : V3{0.0f, 0.0f, 0.0f};
@@ -181,10 +219,11 @@ __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) 
         if (!pinned) acc = {0.0f + s.mass * s.gx, 0.0f + s.mass * s.gy, 0.0f + s.mass * s.gz};
         V3 adv = {s.mass * dvi.x, s.mass * dvi.y, s.mass * dvi.z};
         float C[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        for (int k = 0; k < c.wa; k++) {
-            const int sp = c.ainc_s[k * N + i];
+#pragma unroll
+        for (int k = 0; k < WA; k++) {
+            const int sp = sps[k];
             if (sp < 0) break;
-            const uint16_t m = c.ainc_m[k * N + i];
+            const uint16_t m = ms[k];
             const int j = i + slot_dj(m) * c.nx + slot_di(m);
             V3 xj = ld3(c.x, N, j), vj = ld3(c.v, N, j);
             const bool second = (m >> 8) & 1u;
@@ -232,23 +271,80 @@ __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) 
         t = dot3(r, r);
     }
     float p = tile_sum<NT>(t, ws);
-    if (threadIdx.x == 0) c.del_part[blockIdx.x] = p;
+    if (threadIdx.x == 0) c.del_part[tile] = p;
 }
 
 // ---------------------------------------------------------------------------
 // CG iteration k, part 1: direction update on the tile + halo, q = S A d
 // ---------------------------------------------------------------------------
-template <int TX, int TY>
+template <int TX, int TY, int WP>
 __global__ __launch_bounds__(TX* TY) void k_cg_spmv(DevCloth c, StepScalars s, int k) {
     constexpr int NT = TX * TY;
     constexpr int WX = TX + 2 * H, WY = TY + 2 * H;
+    constexpr int NWIN = (WX * WY + NT - 1) / NT;
     __shared__ float wd[3][WY][WX];
     __shared__ float ws[NT / 64];
     __shared__ float bc;
-    if (c.ctl[0]) return;
     const int nt = c.nt, N = c.N;
-    const float delta = partial_sum(c.del_part + (size_t)k * nt, nt, &bc);
-    const float thresh = s.tol_mode == 1 ? s.tol2 * (k == 0 ? delta : c.scal[0]) : s.tol2;
+    const int tile = tile_of_block(blockIdx.x, nt);
+    const int bx = tile % c.ntx, by = tile / c.ntx;
+    const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+    const int gi = bx * TX + lx, gj = by * TY + ly;
+    const bool own = gi < c.nx && gj < c.ny;
+    const int i = gj * c.nx + gi;
+
+    // ---- loads that do not depend on beta
+    const int done = c.ctl[0];
+    const float delta0 = c.scal[0];
+    const float delta_old = k > 0 ? c.scal[k - 1] : 0.0f;
+    float pv[kPartialRegs];
+    if (threadIdx.x < 64) partial_load(c.del_part + (size_t)k * nt, nt, pv);
+    const float* dprev = (k & 1) ? c.d0 : c.d1;
+    float* dcur = (k & 1) ? c.d1 : c.d0;
+    const int x0 = bx * TX - H, y0 = by * TY - H;
+    V3 wr[NWIN], wp[NWIN];
+#pragma unroll
+    for (int n = 0; n < NWIN; n++) {
+        const int idx = threadIdx.x + n * NT;
+        const int gx = x0 + idx % WX, gy = y0 + idx / WX;
+        wr[n] = wp[n] = V3{0.0f, 0.0f, 0.0f};
+        if (idx < WX * WY && gx >= 0 && gx < c.nx && gy >= 0 && gy < c.ny) {
+            const int g = gy * c.nx + gx;
+            if (k == 0) {
+                wr[n] = ld3(c.d0, N, g);
+            } else {
+                wr[n] = ld3(c.r, N, g);
+                wp[n] = ld3(dprev, N, g);
+            }
+        }
+    }
+    uint16_t m[WP];
+    float C[WP][6];
+#pragma unroll
+    for (int sl = 0; sl < WP; sl++) {
+        m[sl] = (own && sl < c.wp) ? c.pair_m[sl * N + i] : kEmptySlot;
+        if (m[sl] != kEmptySlot) {
+            const int j = i + slot_dj(m[sl]) * c.nx + slot_di(m[sl]);
+            const int blk = ((m[sl] >> 12) & 1) ? i : j;
+            const float* bp = c.blocks + (size_t)((m[sl] >> 8) & 15) * 6 * N + blk;
+#pragma unroll
+            for (int q = 0; q < 6; q++) C[sl][q] = bp[(size_t)q * N];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 6; q++) C[sl][q] = 0.0f;
+        }
+    }
+    const bool pinned = own ? c.pin[i] != 0 : true;
+
+    // ---- r.r of this iteration, stopping test, beta
+    if (threadIdx.x < 64) {
+        const float sum = partial_finish(c.del_part + (size_t)k * nt, nt, pv);
+        if (threadIdx.x == 0) bc = sum;
+    }
+    __syncthreads();
+    if (done) return;
+    const float delta = bc;
+    const float thresh = s.tol_mode == 1 ? s.tol2 * (k == 0 ? delta : delta0) : s.tol2;
     const bool stop = s.tol_mode != 2 && !(delta > thresh);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         c.scal[k] = delta;
@@ -259,61 +355,47 @@ __global__ __launch_bounds__(TX* TY) void k_cg_spmv(DevCloth c, StepScalars s, i
     }
     if (stop) return;
     // beta = 0 once r.r has underflowed to 0 (fixed iteration counts)
-    const float delta_old = k > 0 ? c.scal[k - 1] : 0.0f;
     const float beta = (k > 0 && delta_old != 0.0f) ? delta / delta_old : 0.0f;
-    const float* dprev = (k & 1) ? c.d0 : c.d1;
-    float* dcur = (k & 1) ? c.d1 : c.d0;
-    const int bx = blockIdx.x % c.ntx, by = blockIdx.x / c.ntx;
-    const int x0 = bx * TX - H, y0 = by * TY - H;
-    for (int idx = threadIdx.x; idx < WX * WY; idx += NT) {
-        const int wy = idx / WX, wx = idx % WX;
-        const int gi = x0 + wx, gj = y0 + wy;
-        V3 d = {0.0f, 0.0f, 0.0f};
-        if (gi >= 0 && gi < c.nx && gj >= 0 && gj < c.ny) {
-            const int i = gj * c.nx + gi;
-            if (k == 0) {
-                d = ld3(c.d0, N, i);
-            } else {
-                V3 r = ld3(c.r, N, i), dp = ld3(dprev, N, i);
-                d = {r.x + beta * dp.x, r.y + beta * dp.y, r.z + beta * dp.z};
-                if (wx >= H && wx < H + TX && wy >= H && wy < H + TY) st3(dcur, N, i, d);
+
+    // ---- d = r + beta d on tile + halo into LDS; the tile's own d to memory
+#pragma unroll
+    for (int n = 0; n < NWIN; n++) {
+        const int idx = threadIdx.x + n * NT;
+        if (idx < WX * WY) {
+            const int wx = idx % WX, wy = idx / WX;
+            V3 d = wr[n];
+            if (k > 0) {
+                d = {wr[n].x + beta * wp[n].x, wr[n].y + beta * wp[n].y, wr[n].z + beta * wp[n].z};
+                const int gx = x0 + wx, gy = y0 + wy;
+                if (wx >= H && wx < H + TX && wy >= H && wy < H + TY && gx < c.nx && gy < c.ny)
+                    st3(dcur, N, gy * c.nx + gx, d);
             }
+            wd[0][wy][wx] = d.x;
+            wd[1][wy][wx] = d.y;
+            wd[2][wy][wx] = d.z;
         }
-        wd[0][wy][wx] = d.x;
-        wd[1][wy][wx] = d.y;
-        wd[2][wy][wx] = d.z;
     }
     __syncthreads();
-    const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
-    const int gi = bx * TX + lx, gj = by * TY + ly;
+
+    // ---- q = S A d
     float t = 0.0f;
-    if (gi < c.nx && gj < c.ny) {
-        const int i = gj * c.nx + gi;
+    if (own) {
         const V3 di = {wd[0][ly + H][lx + H], wd[1][ly + H][lx + H], wd[2][ly + H][lx + H]};
         V3 acc = {s.mass * di.x, s.mass * di.y, s.mass * di.z};
-        for (int sl = 0; sl < c.wp; sl++) {
-            const uint16_t m = c.pair_m[sl * N + i];
-            if (m == kEmptySlot) break;
-            const int ddi = slot_di(m), ddj = slot_dj(m);
-            const int plane = (m >> 8) & 15;
-            const int owner = (m >> 12) & 1;
-            const int j = i + ddj * c.nx + ddi;
-            const int blk = owner ? i : j;
-            const float* bp = c.blocks + (size_t)plane * 6 * N + blk;
-            float C[6];
 #pragma unroll
-            for (int q = 0; q < 6; q++) C[q] = bp[(size_t)q * N];
-            const int wy = ly + H + ddj, wx = lx + H + ddi;
+        for (int sl = 0; sl < WP; sl++) {
+            if (m[sl] == kEmptySlot) break;
+            const int wy = ly + H + slot_dj(m[sl]), wx = lx + H + slot_di(m[sl]);
             V3 w = {wd[0][wy][wx] - di.x, wd[1][wy][wx] - di.y, wd[2][wy][wx] - di.z};
-            V3 a = symmv(C, w);
+            V3 a = symmv(C[sl], w);
             acc = {acc.x + a.x, acc.y + a.y, acc.z + a.z};
         }
-        V3 q = c.pin[i] ? V3{0.0f, 0.0f, 0.0f} : acc;
+        V3 q = pinned ? V3{0.0f, 0.0f, 0.0f} : acc;
         st3(c.q, N, i, q);
         t = dot3(di, q);
     }
     float p = tile_sum<NT>(t, ws);
-    if (threadIdx.x == 0) c.sig_part[(size_t)k * nt + blockIdx.x] = p;
+    if (threadIdx.x == 0) c.sig_part[(size_t)k * nt + tile] = p;
 }
 
 // CG iteration k, part 2: dv += alpha d, r -= alpha q, partial r.r
@@ -322,10 +404,34 @@ __global__ __launch_bounds__(TX* TY) void k_cg_axpy(DevCloth c, StepScalars s, i
     constexpr int NT = TX * TY;
     __shared__ float ws[NT / 64];
     __shared__ float bc;
-    if (c.ctl[0]) return;
     const int nt = c.nt, N = c.N;
-    const float sigma = partial_sum(c.sig_part + (size_t)k * nt, nt, &bc);
+    const int tile = tile_of_block(blockIdx.x, nt);
+    const int bx = tile % c.ntx, by = tile / c.ntx;
+    const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+    const int gi = bx * TX + lx, gj = by * TY + ly;
+    const bool own = gi < c.nx && gj < c.ny;
+    const int i = gj * c.nx + gi;
+
+    // ---- loads that do not depend on alpha
+    const int done = c.ctl[0];
     const float delta = c.scal[k];
+    float pv[kPartialRegs];
+    if (threadIdx.x < 64) partial_load(c.sig_part + (size_t)k * nt, nt, pv);
+    const float* dcur = (k & 1) ? c.d1 : c.d0;
+    V3 d = {0.0f, 0.0f, 0.0f}, q = d, x = d, r = d;
+    if (own) {
+        d = ld3(dcur, N, i);
+        q = ld3(c.q, N, i);
+        x = ld3(c.dv, N, i);
+        r = ld3(c.r, N, i);
+    }
+    if (threadIdx.x < 64) {
+        const float sum = partial_finish(c.sig_part + (size_t)k * nt, nt, pv);
+        if (threadIdx.x == 0) bc = sum;
+    }
+    __syncthreads();
+    if (done) return;
+    const float sigma = bc;
     if (sigma < 0.0f || !isfinite(sigma)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             c.ctl[0] = 1;
@@ -336,14 +442,8 @@ __global__ __launch_bounds__(TX* TY) void k_cg_axpy(DevCloth c, StepScalars s, i
     }
     // sigma == 0 only after r (hence d) underflowed: no update, no 0/0
     const float alpha = sigma == 0.0f ? 0.0f : delta / sigma;
-    const float* dcur = (k & 1) ? c.d1 : c.d0;
-    const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
-    const int bx = blockIdx.x % c.ntx, by = blockIdx.x / c.ntx;
-    const int gi = bx * TX + lx, gj = by * TY + ly;
     float t = 0.0f;
-    if (gi < c.nx && gj < c.ny) {
-        const int i = gj * c.nx + gi;
-        V3 d = ld3(dcur, N, i), q = ld3(c.q, N, i), x = ld3(c.dv, N, i), r = ld3(c.r, N, i);
+    if (own) {
         x = {x.x + alpha * d.x, x.y + alpha * d.y, x.z + alpha * d.z};
         r = {r.x - alpha * q.x, r.y - alpha * q.y, r.z - alpha * q.z};
         st3(c.dv, N, i, x);
@@ -351,7 +451,7 @@ __global__ __launch_bounds__(TX* TY) void k_cg_axpy(DevCloth c, StepScalars s, i
         t = dot3(r, r);
     }
     float p = tile_sum<NT>(t, ws);
-    if (threadIdx.x == 0) c.del_part[(size_t)(k + 1) * nt + blockIdx.x] = p;
+    if (threadIdx.x == 0) c.del_part[(size_t)(k + 1) * nt + tile] = p;
 }
 
 // Iteration count and status of the solve (one 64-thread block).
@@ -407,14 +507,22 @@ hipError_t launch_forces(const DevCloth& c, const StepScalars& s, hipStream_t st
 }
 
 hipError_t launch_assemble(const DevCloth& c, const StepScalars& s, hipStream_t st) {
-    hipLaunchKernelGGL((k_assemble<kTileX, kTileY>), dim3(c.nt), dim3(kTileX * kTileY), 0, st, c,
-                       s);
+    if (c.wa <= 16)
+        hipLaunchKernelGGL((k_assemble<kTileX, kTileY, 16>), dim3(c.nt), dim3(kTileX * kTileY), 0,
+                           st, c, s);
+    else
+        hipLaunchKernelGGL((k_assemble<kTileX, kTileY, kMaxAssemblyWidth>), dim3(c.nt),
+                           dim3(kTileX * kTileY), 0, st, c, s);
     return hipGetLastError();
 }
 
 hipError_t launch_cg_spmv(const DevCloth& c, const StepScalars& s, int k, hipStream_t st) {
-    hipLaunchKernelGGL((k_cg_spmv<kTileX, kTileY>), dim3(c.nt), dim3(kTileX * kTileY), 0, st, c,
-                       s, k);
+    if (c.wp <= 12)
+        hipLaunchKernelGGL((k_cg_spmv<kTileX, kTileY, 12>), dim3(c.nt), dim3(kTileX * kTileY), 0,
+                           st, c, s, k);
+    else
+        hipLaunchKernelGGL((k_cg_spmv<kTileX, kTileY, kMaxPairWidth>), dim3(c.nt),
+                           dim3(kTileX * kTileY), 0, st, c, s, k);
     return hipGetLastError();
 }
 
@@ -444,4 +552,3 @@ hipError_t launch_update(const DevCloth& c, const StepScalars& s, hipStream_t st
 }
 
 }  // namespace oc
-
