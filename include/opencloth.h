@@ -161,6 +161,50 @@ int oc_cloth_get_pairs(oc_cloth cloth, int32_t* a, int32_t* b);
  * rows zero (host AoS); either may be NULL; synchronises. */
 int oc_cloth_get_system(oc_cloth cloth, float* blocks, float* rhs);
 
+/* ---- batched cloths ---------------------------------------------------- */
+/* Many independent cloths of uneven sizes stepped together: one persistent
+ * kernel launch per step runs assembly, filtered CG (per-cloth stopping
+ * test, converged cloths masked out) and the update for every cloth.  No
+ * counterpart in the reference, whose demos each step one global cloth; each
+ * cloth follows the same step as oc_cloth_step. */
+typedef struct oc_batch_s* oc_batch;
+
+/* Schedule of the persistent kernel: workgroup size (256, 512 or 1024) and
+ * the target vertices per thread, which sets the threads serving one cloth
+ * (a power of two from 64 to block_threads); cloths served by fewer threads
+ * than a workgroup are packed several to a workgroup. */
+typedef struct {
+    int32_t block_threads;
+    int32_t vertices_per_thread;
+} oc_batch_schedule;
+
+/* n_cloths grids of nx[c]*ny[c] vertices.  x0/v0/pinned: all cloths'
+ * vertices concatenated in cloth order (AoS float3 / bytes; v0 and pinned may
+ * be NULL).  springs: all cloths' spring lists concatenated, n_springs[c]
+ * records for cloth c, whose p1/p2 index that cloth's own vertices. */
+int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, const float* x0,
+                    const float* v0, const oc_spring* springs, const int32_t* n_springs,
+                    const uint8_t* pinned, const oc_params* params, oc_batch* out);
+int oc_batch_destroy(oc_batch batch);
+int oc_batch_set_stream(oc_batch batch, void* hip_stream);
+int oc_batch_set_params(oc_batch batch, const oc_params* params);
+int oc_batch_set_schedule(oc_batch batch, const oc_batch_schedule* schedule);
+int oc_batch_get_schedule(oc_batch batch, oc_batch_schedule* schedule);
+/* Time every candidate schedule on one step (state saved and restored),
+ * keep the fastest and return it in *best (may be NULL); synchronises. */
+int oc_batch_autotune(oc_batch batch, oc_batch_schedule* best);
+/* Threads that serve each cloth under the current schedule (n_cloths ints). */
+int oc_batch_threads_per_cloth(oc_batch batch, int32_t* threads);
+int oc_batch_set_state(oc_batch batch, const float* x, const float* v);
+int oc_batch_get_state(oc_batch batch, float* x, float* v);
+/* nsteps steps, each a replay of the captured HIP graph of one step. */
+int oc_batch_step(oc_batch batch, int32_t nsteps);
+/* nsteps steps launched directly (no graph). */
+int oc_batch_step_eager(oc_batch batch, int32_t nsteps);
+/* CG iterations and OC_SOLVE_* bits of every cloth's last step; synchronises. */
+int oc_batch_get_status(oc_batch batch, int32_t* iters, int32_t* status);
+int oc_batch_synchronize(oc_batch batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,6 +49,20 @@ SIGNATURES = {
     "oc_cloth_num_pairs": (C.c_int, [_P, _P]),
     "oc_cloth_get_pairs": (C.c_int, [_P, _P, _P]),
     "oc_cloth_get_system": (C.c_int, [_P, _P, _P]),
+    "oc_batch_create": (C.c_int, [_I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "oc_batch_destroy": (C.c_int, [_P]),
+    "oc_batch_set_stream": (C.c_int, [_P, _P]),
+    "oc_batch_set_params": (C.c_int, [_P, _P]),
+    "oc_batch_set_schedule": (C.c_int, [_P, _P]),
+    "oc_batch_get_schedule": (C.c_int, [_P, _P]),
+    "oc_batch_autotune": (C.c_int, [_P, _P]),
+    "oc_batch_threads_per_cloth": (C.c_int, [_P, _P]),
+    "oc_batch_set_state": (C.c_int, [_P, _P, _P]),
+    "oc_batch_get_state": (C.c_int, [_P, _P, _P]),
+    "oc_batch_step": (C.c_int, [_P, _I]),
+    "oc_batch_step_eager": (C.c_int, [_P, _I]),
+    "oc_batch_get_status": (C.c_int, [_P, _P, _P]),
+    "oc_batch_synchronize": (C.c_int, [_P]),
 }
 
 
@@ -60,6 +74,12 @@ class OcParams(C.Structure):
         ("cg_max_iters", C.c_int32), ("cg_tol", C.c_float), ("cg_tol_mode", C.c_int32),
         ("warm_start", C.c_int32), ("flags", C.c_int32),
     ]
+
+
+class OcBatchSchedule(C.Structure):
+    """oc_batch_schedule (include/opencloth.h)."""
+
+    _fields_ = [("block_threads", C.c_int32), ("vertices_per_thread", C.c_int32)]
 
 
 class OpenClothError(RuntimeError):

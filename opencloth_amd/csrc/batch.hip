@@ -1,0 +1,289 @@
+// Persistent batched implicit step (gfx950, wave64).
+//
+// One launch steps every cloth of a batch.  Workgroups pull work items (packs
+// of similar-sized cloths, largest first) from a queue; the threads of a
+// workgroup are split evenly over the pack's cloths and each cloth's
+// assembly, filtered CG and update run inside its workgroup, so no step needs
+// a grid-wide barrier.  Cloths that meet their stopping test are masked out
+// while the rest of the pack iterates.  The loop exits when the queue is
+// drained, which every workgroup reaches.
+//
+// Per cloth the arithmetic is the single-cloth step's (same per-vertex
+// gathers, pair order and expressions).  Reductions: thread t of a cloth's T
+// threads sums vertices t, t+T, ... in order, then the 64-lane wave tree,
+// then the cloth's waves in order.
+#include "oc_device.hpp"
+
+namespace oc {
+
+namespace {
+
+using namespace dev;
+
+// Sum over the T threads serving each cloth; valid where threadIdx.x % T == 0.
+template <int NW>
+__device__ inline float seg_sum(float t, int T, float* ws) {
+    t = wave_sum(t);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = t;
+    __syncthreads();
+    float p = 0.0f;
+    if (threadIdx.x % T == 0) {
+        const int w0 = threadIdx.x >> 6, nw = T >> 6;
+        p = ws[w0];
+        for (int k = 1; k < nw; k++) p = p + ws[w0 + k];
+    }
+    __syncthreads();
+    return p;
+}
+
+// Assembly of vertex i (global index) of a cloth with row length nx:
+// pair blocks it owns, rhs b, CG start r = d = S(b - A dv); returns r.r.
+__device__ inline float assemble_vertex(const DevBatch& c, const StepScalars& s, int i, int nx) {
+    const int N = c.Ntot;
+    const float h = s.h, hh = s.hh;
+    const bool pinned = c.pin[i] != 0;
+    V3 xi = ld3(c.x, N, i), vi = ld3(c.v, N, i);
+    V3 dvi = s.warm ? ld3(c.dv, N, i) : V3{0.0f, 0.0f, 0.0f};
+    if (!s.warm) st3(c.dv, N, i, dvi);
+    V3 acc = {0.0f, 0.0f, 0.0f}, kv = {0.0f, 0.0f, 0.0f};
+    if (!pinned) acc = {0.0f + s.mass * s.gx, 0.0f + s.mass * s.gy, 0.0f + s.mass * s.gz};
+    V3 adv = {s.mass * dvi.x, s.mass * dvi.y, s.mass * dvi.z};
+    float C[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k = 0; k < c.wa; k++) {
+        const int sp = c.ainc_s[(size_t)k * N + i];
+        if (sp < 0) break;
+        const uint16_t m = c.ainc_m[(size_t)k * N + i];
+        const int j = i + slot_dj(m) * nx + slot_di(m);
+        V3 xj = ld3(c.x, N, j), vj = ld3(c.v, N, j);
+        const bool second = (m >> 8) & 1u;
+        V3 f;
+        float K[6], P[6];
+        const float kd = c.kd[sp];
+        if (second)
+            spring_terms(xj, xi, vj, vi, c.L0[sp], c.ks[sp], kd, f, K, P);
+        else
+            spring_terms(xi, xj, vi, vj, c.L0[sp], c.ks[sp], kd, f, K, P);
+#pragma unroll
+        for (int q = 0; q < 6; q++) C[q] += h * (kd * P[q]) + hh * K[q];
+        V3 w = second ? sub3(vj, vi) : sub3(vi, vj);
+        V3 kw = symmv(K, w);
+        if (second) {
+            acc = sub3(acc, f);
+            kv = sub3(kv, kw);
+        } else {
+            acc = add3(acc, f);
+            kv = add3(kv, kw);
+        }
+        if ((m >> 13) & 1u) {
+            if ((m >> 14) & 1u) {
+                const int plane = (m >> 9) & 15;
+#pragma unroll
+                for (int q = 0; q < 6; q++) c.blocks[(size_t)(plane * 6 + q) * N + i] = C[q];
+            }
+            V3 dvj = s.warm ? ld3(c.dv, N, j) : V3{0.0f, 0.0f, 0.0f};
+            adv = add3(adv, symmv(C, sub3(dvj, dvi)));
+#pragma unroll
+            for (int q = 0; q < 6; q++) C[q] = 0.0f;
+        }
+    }
+    V3 b;
+    b.x = pinned ? 0.0f : h * (acc.x + h * kv.x);
+    b.y = pinned ? 0.0f : h * (acc.y + h * kv.y);
+    b.z = pinned ? 0.0f : h * (acc.z + h * kv.z);
+    st3(c.b, N, i, b);
+    V3 r = pinned ? V3{0.0f, 0.0f, 0.0f} : sub3(b, adv);
+    st3(c.r, N, i, r);
+    st3(c.d, N, i, r);
+    return dot3(r, r);
+}
+
+// q_i = S (A d)_i; returns d_i . q_i
+__device__ inline float spmv_vertex(const DevBatch& c, const StepScalars& s, int i, int nx) {
+    const int N = c.Ntot;
+    const V3 di = ld3(c.d, N, i);
+    V3 acc = scl3(s.mass, di);
+    for (int sl = 0; sl < c.wp; sl++) {
+        const uint16_t m = c.pair_m[(size_t)sl * N + i];
+        if (m == kEmptySlot) break;
+        const int j = i + slot_dj(m) * nx + slot_di(m);
+        const int blk = ((m >> 12) & 1) ? i : j;
+        const float* bp = c.blocks + (size_t)((m >> 8) & 15) * 6 * N + blk;
+        float C[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) C[q] = bp[(size_t)q * N];
+        acc = add3(acc, symmv(C, sub3(ld3(c.d, N, j), di)));
+    }
+    const V3 q = c.pin[i] ? V3{0.0f, 0.0f, 0.0f} : acc;
+    st3(c.q, N, i, q);
+    return dot3(di, q);
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_batch_step(DevBatch c, StepScalars s) {
+    constexpr int NW = NT / 64;
+    __shared__ float ws[NW];
+    __shared__ float sd[NW], sd_old[NW], sd0[NW], ssig[NW];
+    __shared__ int sdone[NW], sit[NW], sst[NW];
+    __shared__ int sitem;
+    const int tid = threadIdx.x;
+    const int N = c.Ntot;
+    for (;;) {
+        if (tid == 0) sitem = atomicAdd(c.queue, 1);
+        __syncthreads();
+        const int item = sitem;
+        __syncthreads();
+        if (item >= c.n_items) return;
+        const int T = c.item_T[item], G = c.item_count[item];
+        const int g = tid / T, lt = tid % T;
+        const bool act = g < G;
+        int ins = 0, voff = 0, nx = 1, Ni = 0;
+        if (act) {
+            ins = c.order[c.item_inst[item] + g];
+            voff = c.inst_voff[ins];
+            nx = c.inst_nx[ins];
+            Ni = c.inst_n[ins];
+        }
+
+        // ---- assembly and CG start
+        float t = 0.0f;
+        for (int vv = lt; vv < Ni; vv += T) t += assemble_vertex(c, s, voff + vv, nx);
+        float tot = seg_sum<NW>(t, T, ws);
+        if (act && lt == 0) {
+            sd[g] = tot;
+            sd0[g] = tot;
+            sd_old[g] = 0.0f;
+            sdone[g] = 0;
+            sit[g] = 0;
+            sst[g] = 0;
+        }
+        __syncthreads();
+
+        // ---- filtered CG, converged cloths masked
+        for (int k = 0; k < s.max_it; k++) {
+            if (act && lt == 0 && !sdone[g]) {
+                const float thresh = s.tol_mode == 1 ? s.tol2 * sd0[g] : s.tol2;
+                if (s.tol_mode != 2 && !(sd[g] > thresh)) {
+                    sdone[g] = 1;
+                    sit[g] = k;
+                }
+            }
+            __syncthreads();
+            bool all = true;
+            for (int gg = 0; gg < G; gg++) all = all && sdone[gg];
+            if (all) break;
+            const bool live = act && !sdone[g];
+            if (live && k > 0) {
+                const float beta = sd_old[g] != 0.0f ? sd[g] / sd_old[g] : 0.0f;
+                for (int vv = lt; vv < Ni; vv += T) {
+                    const int i = voff + vv;
+                    V3 r = ld3(c.r, N, i), dp = ld3(c.d, N, i);
+                    st3(c.d, N, i, V3{r.x + beta * dp.x, r.y + beta * dp.y, r.z + beta * dp.z});
+                }
+            }
+            __syncthreads();
+            t = 0.0f;
+            if (live)
+                for (int vv = lt; vv < Ni; vv += T) t += spmv_vertex(c, s, voff + vv, nx);
+            tot = seg_sum<NW>(t, T, ws);
+            if (act && lt == 0) ssig[g] = tot;
+            __syncthreads();
+            bool bd = false;
+            float alpha = 0.0f;
+            if (live) {
+                const float sigma = ssig[g];
+                if (sigma < 0.0f || !isfinite(sigma))
+                    bd = true;
+                else
+                    alpha = sigma == 0.0f ? 0.0f : sd[g] / sigma;
+            }
+            __syncthreads();
+            if (live && bd && lt == 0) {
+                sdone[g] = 1;
+                sit[g] = k;
+                sst[g] |= OC_SOLVE_BREAKDOWN;
+            }
+            t = 0.0f;
+            if (live && !bd)
+                for (int vv = lt; vv < Ni; vv += T) {
+                    const int i = voff + vv;
+                    V3 d = ld3(c.d, N, i), q = ld3(c.q, N, i), x = ld3(c.dv, N, i),
+                       r = ld3(c.r, N, i);
+                    x = {x.x + alpha * d.x, x.y + alpha * d.y, x.z + alpha * d.z};
+                    r = {r.x - alpha * q.x, r.y - alpha * q.y, r.z - alpha * q.z};
+                    st3(c.dv, N, i, x);
+                    st3(c.r, N, i, r);
+                    t += dot3(r, r);
+                }
+            tot = seg_sum<NW>(t, T, ws);
+            if (live && !bd && lt == 0) {
+                sd_old[g] = sd[g];
+                sd[g] = tot;
+            }
+            __syncthreads();
+        }
+
+        // ---- status and update
+        if (act && lt == 0) {
+            const int it = sdone[g] ? sit[g] : s.max_it;
+            const float delta = sd[g];
+            const float thresh = s.tol_mode == 1 ? s.tol2 * sd0[g] : s.tol2;
+            int st = sst[g];
+            if (s.tol_mode != 2 && delta > thresh && !(st & OC_SOLVE_BREAKDOWN))
+                st |= OC_SOLVE_NOT_CONVERGED;
+            if (!isfinite(delta)) st |= OC_SOLVE_NONFINITE;
+            sit[g] = it;
+            sst[g] = st;
+        }
+        __syncthreads();
+        bool bad = false;
+        for (int vv = lt; vv < Ni; vv += T) {
+            const int i = voff + vv;
+            V3 v = ld3(c.v, N, i), dv = ld3(c.dv, N, i), x = ld3(c.x, N, i);
+            v = add3(v, dv);
+            x = {x.x + s.h * v.x, x.y + s.h * v.y, x.z + s.h * v.z};
+            if ((s.flags & OC_FLAG_GROUND_CLAMP) && x.y < 0.0f) x.y = 0.0f;
+            st3(c.v, N, i, v);
+            st3(c.x, N, i, x);
+            bad = bad || !(isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) &&
+                           isfinite(v.y) && isfinite(v.z));
+        }
+        if (bad) atomicOr(&sst[g], OC_SOLVE_NONFINITE);
+        __syncthreads();
+        if (act && lt == 0) {
+            c.inst_iters[ins] = sit[g];
+            c.inst_status[ins] = sst[g];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_batch_reset(int* queue) {
+    if (threadIdx.x == 0) *queue = 0;
+}
+
+}  // namespace
+
+hipError_t launch_batch_reset(const DevBatch& c, hipStream_t st) {
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, c.queue);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_step(const DevBatch& c, const StepScalars& s, int block_threads, int grid,
+                             hipStream_t st) {
+    switch (block_threads) {
+        case 256:
+            hipLaunchKernelGGL((k_batch_step<256>), dim3(grid), dim3(256), 0, st, c, s);
+            break;
+        case 512:
+            hipLaunchKernelGGL((k_batch_step<512>), dim3(grid), dim3(512), 0, st, c, s);
+            break;
+        case 1024:
+            hipLaunchKernelGGL((k_batch_step<1024>), dim3(grid), dim3(1024), 0, st, c, s);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace oc

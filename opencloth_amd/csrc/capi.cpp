@@ -32,6 +32,11 @@ size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
 }  // namespace
 
+namespace oc {
+// Records the message returned by oc_last_error on this thread.
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace oc
+
 struct oc_cloth_s {
     oc::GridTopology topo;
     oc_params prm{};

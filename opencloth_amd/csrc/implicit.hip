@@ -26,123 +26,14 @@
 // Every CG kernel issues all loads that do not depend on the previous
 // reduction (its own vectors, pair blocks, the partials) before waiting, so
 // the cross-tile reduction overlaps the memory traffic.
-#include "oc_internal.hpp"
+#include "oc_device.hpp"
 
 namespace oc {
 
 namespace {
 
 constexpr int H = kHalo;
-constexpr int kPartialRegs = 16;  // partials per lane loaded up front (1024 tiles)
-
-struct V3 {
-    float x, y, z;
-};
-
-__device__ inline V3 ld3(const float* p, int N, int i) { return {p[i], p[N + i], p[2 * N + i]}; }
-__device__ inline void st3(float* p, int N, int i, V3 v) {
-    p[i] = v.x;
-    p[N + i] = v.y;
-    p[2 * N + i] = v.z;
-}
-__device__ inline float dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-// symmetric (xx,yy,zz,xy,xz,yz) times vector, mat3*vec3 order
-__device__ inline V3 symmv(const float* c, V3 w) {
-    return {c[0] * w.x + c[3] * w.y + c[4] * w.z, c[3] * w.x + c[1] * w.y + c[5] * w.z,
-            c[4] * w.x + c[5] * w.y + c[2] * w.z};
-}
-
-// Force on p1 (IE:663-675), stiffness Jacobian K (IE:669-671) and
-// P = dp dp^T / l^2, in the reference's expression order.
-__device__ inline void spring_terms(V3 x1, V3 x2, V3 v1, V3 v2, float L0, float Ks, float Kd,
-                                    V3& f, float* K, float* P) {
-    V3 dp = {x1.x - x2.x, x1.y - x2.y, x1.z - x2.z};
-    V3 dv = {v1.x - v2.x, v1.y - v2.y, v1.z - v2.z};
-    float sq = dp.x * dp.x + dp.y * dp.y + dp.z * dp.z;
-    float dist = sqrtf(sq);
-    float dist2 = dist * dist;
-    float lo_l = L0 / dist;
-    P[0] = (dp.x * dp.x) / dist2;
-    P[1] = (dp.y * dp.y) / dist2;
-    P[2] = (dp.z * dp.z) / dist2;
-    P[3] = (dp.y * dp.x) / dist2;
-    P[4] = (dp.z * dp.x) / dist2;
-    P[5] = (dp.z * dp.y) / dist2;
-#pragma unroll
-    for (int c = 0; c < 3; c++) K[c] = Ks * (-1.0f + lo_l * (1.0f - P[c]));
-#pragma unroll
-    for (int c = 3; c < 6; c++) K[c] = Ks * (lo_l * (0.0f - P[c]));
-    float leftTerm = -Ks * (dist - L0);
-    float rightTerm = Kd * ((dv.x * dp.x + dv.y * dp.y + dv.z * dp.z) / dist);
-    float inv = 1.0f / sqrtf(sq);
-    float s = leftTerm + rightTerm;
-    f = {s * (dp.x * inv), s * (dp.y * inv), s * (dp.z * inv)};
-}
-
-// Force only (no Jacobian), same arithmetic as spring_terms.
-__device__ inline V3 spring_force(V3 x1, V3 x2, V3 v1, V3 v2, float L0, float Ks, float Kd) {
-    V3 dp = {x1.x - x2.x, x1.y - x2.y, x1.z - x2.z};
-    V3 dv = {v1.x - v2.x, v1.y - v2.y, v1.z - v2.z};
-    float sq = dp.x * dp.x + dp.y * dp.y + dp.z * dp.z;
-    float dist = sqrtf(sq);
-    float leftTerm = -Ks * (dist - L0);
-    float rightTerm = Kd * ((dv.x * dp.x + dv.y * dp.y + dv.z * dp.z) / dist);
-    float inv = 1.0f / sqrtf(sq);
-    float s = leftTerm + rightTerm;
-    return {s * (dp.x * inv), s * (dp.y * inv), s * (dp.z * inv)};
-}
-
-// Wave tree: lane l += lane l+off for off = 32..1; lane 0 ends with the sum.
-__device__ inline float wave_sum(float t) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) t += __shfl_down(t, off, 64);
-    return t;
-}
-
-// Tile partial: wave sums added in wave order.  Result valid in thread 0.
-template <int NT>
-__device__ inline float tile_sum(float t, float* ws) {
-    t = wave_sum(t);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) ws[w] = t;
-    __syncthreads();
-    float p = 0.0f;
-    if (threadIdx.x == 0) {
-        p = ws[0];
-#pragma unroll
-        for (int k = 1; k < NT / 64; k++) p = p + ws[k];
-    }
-    return p;
-}
-
-// Cross-tile sum in two halves so the loads can be issued early.  Lane l of
-// wave 0 sums tiles l, l+64, l+128, ... in order, then the wave tree.
-__device__ inline void partial_load(const float* part, int nt, float* v) {
-#pragma unroll
-    for (int j = 0; j < kPartialRegs; j++) {
-        const int k = threadIdx.x + 64 * j;
-        v[j] = k < nt ? part[k] : 0.0f;
-    }
-}
-__device__ inline float partial_finish(const float* part, int nt, const float* v) {
-    float s = 0.0f;
-#pragma unroll
-    for (int j = 0; j < kPartialRegs; j++) s += v[j];
-    for (int k = threadIdx.x + 64 * kPartialRegs; k < nt; k += 64) s += part[k];
-    return wave_sum(s);
-}
-
-// Sum of nt tile partials, returned to all threads (one-shot form).
-__device__ inline float partial_sum(const float* part, int nt, float* bc) {
-    if (threadIdx.x < 64) {
-        float v[kPartialRegs];
-        partial_load(part, nt, v);
-        const float s = partial_finish(part, nt, v);
-        if (threadIdx.x == 0) *bc = s;
-    }
-    __syncthreads();
-    return *bc;
-}
+using namespace dev;
 
 // Blocks are dispatched round-robin over the 8 XCDs; each XCD gets a
 // contiguous band of tiles so that halo and shared-block re-reads hit its L2.

@@ -94,4 +94,42 @@ hipError_t launch_update(const DevCloth& c, const StepScalars& s, hipStream_t st
 constexpr int kTileX = 32;
 constexpr int kTileY = 8;
 
+// ---- batched cloths --------------------------------------------------------
+// All cloths' vertices are concatenated (SoA planes of stride Ntot); slot
+// tables are slot-major with stride Ntot and hold global spring ids.  A work
+// item is a pack of `count` consecutive cloths (in the size-sorted order)
+// served by T threads each.
+struct DevBatch {
+    int n_items, Ntot, wa, wp;
+    float* x;
+    float* v;
+    float* dv;
+    float* r;
+    float* d;
+    float* q;
+    float* b;
+    const uint8_t* pin;
+    float* blocks;  // n_planes*6*Ntot
+    const int32_t* ainc_s;
+    const uint16_t* ainc_m;
+    const uint16_t* pair_m;
+    const float* L0;
+    const float* ks;
+    const float* kd;
+    const int* inst_voff;
+    const int* inst_nx;
+    const int* inst_n;
+    const int* item_inst;   // per item: first entry in `order`
+    const int* item_count;  // cloths in the item
+    const int* item_T;      // threads per cloth
+    const int* order;       // cloth ids sorted by size (descending)
+    int* queue;
+    int* inst_iters;
+    int* inst_status;
+};
+
+hipError_t launch_batch_reset(const DevBatch& c, hipStream_t st);
+hipError_t launch_batch_step(const DevBatch& c, const StepScalars& s, int block_threads,
+                             int grid, hipStream_t st);
+
 }  // namespace oc

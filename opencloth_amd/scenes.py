@@ -202,3 +202,74 @@ def large_scene(n: int = 512, steps: int = 100, seed: int = 0) -> Scene:
     pinned[:n] = 1
     p = StepParams(dt=1.0 / 120.0, cg_max_iters=30, cg_tol=0.0, cg_tol_mode=TOL_NONE)
     return Scene(f"large_{n}x{n}", n, n, X, np.zeros_like(X), springs, pinned, p, steps)
+
+
+@dataclass
+class BatchScene:
+    name: str
+    nx: np.ndarray  # (C,) int32
+    ny: np.ndarray
+    X: np.ndarray  # (sum N, 3) float32, cloths concatenated
+    V: np.ndarray
+    springs: np.ndarray  # concatenated SPRING_DTYPE, per-cloth local indices
+    n_springs: np.ndarray  # (C,) int32
+    pinned: np.ndarray  # (sum N,) uint8
+    params: StepParams
+    steps: int
+
+    @property
+    def n_cloths(self) -> int:
+        return len(self.nx)
+
+    @property
+    def voff(self) -> np.ndarray:
+        n = (self.nx.astype(np.int64) * self.ny)
+        return np.concatenate([[0], np.cumsum(n)[:-1]])
+
+    @property
+    def soff(self) -> np.ndarray:
+        return np.concatenate([[0], np.cumsum(self.n_springs.astype(np.int64))[:-1]])
+
+    def cloth(self, c: int) -> Scene:
+        """Cloth c as a single-cloth Scene."""
+        v0, s0 = int(self.voff[c]), int(self.soff[c])
+        n = int(self.nx[c] * self.ny[c])
+        return Scene(f"{self.name}[{c}]", int(self.nx[c]), int(self.ny[c]), self.X[v0:v0 + n],
+                     self.V[v0:v0 + n], self.springs[s0:s0 + int(self.n_springs[c])],
+                     self.pinned[v0:v0 + n], self.params, self.steps)
+
+
+def batch_scene(n_cloths: int = 4096, sides=(16, 128), seed: int = 0, steps: int = 200,
+                ks_range=(20.0, 200.0), kd_range=(0.1, 0.5), pins=(2, 8),
+                dt: float = 1.0 / 60.0, cg_max_iters: int = 40, cg_tol: float = 1e-4) -> BatchScene:
+    """n_cloths independent square cloths over the 4x4 square at y=0; side
+    uniform in [sides], ks log-uniform in ks_range and kd uniform in kd_range
+    per cloth, `pins` random top-edge (row 0) vertices pinned per cloth; CG at
+    most cg_max_iters iterations with per-cloth relative tolerance cg_tol."""
+    u = uniform(seed, 4 * n_cloths).reshape(4, n_cloths)
+    side = (sides[0] + np.floor(u[0] * (sides[1] - sides[0] + 1))).astype(np.int32)
+    ks = np.exp(np.log(ks_range[0]) + u[1] * (np.log(ks_range[1]) - np.log(ks_range[0])))
+    kd = kd_range[0] + u[2] * (kd_range[1] - kd_range[0])
+    npin = (pins[0] + np.floor(u[3] * (pins[1] - pins[0] + 1))).astype(np.int32)
+    cache = {}  # side -> (positions, springs): the topology depends on the side only
+    for sd in np.unique(side):
+        X0 = grid_positions(int(sd), int(sd), 4.0, 0.0)
+        cache[int(sd)] = (X0, grid_springs(int(sd), int(sd), X0))
+    nverts = side.astype(np.int64) ** 2
+    nspr = np.array([len(cache[int(sd)][1]) for sd in side], np.int64)
+    voff = np.concatenate([[0], np.cumsum(nverts)])
+    soff = np.concatenate([[0], np.cumsum(nspr)])
+    X = np.empty((int(voff[-1]), 3), np.float32)
+    springs = np.empty(int(soff[-1]), SPRING_DTYPE)
+    pinned = np.zeros(int(voff[-1]), np.uint8)
+    for c in range(n_cloths):
+        X0, sp0 = cache[int(side[c])]
+        X[voff[c]:voff[c + 1]] = X0
+        springs[soff[c]:soff[c + 1]] = sp0
+        perm = np.argsort(uniform(seed * 1000003 + c + 1, int(side[c])))
+        pinned[voff[c] + perm[: int(npin[c])]] = 1
+    springs["Ks"] = np.repeat(ks.astype(np.float32), nspr)
+    springs["Kd"] = np.repeat(-kd.astype(np.float32), nspr)
+    p = StepParams(dt=dt, cg_max_iters=cg_max_iters, cg_tol=cg_tol, cg_tol_mode=TOL_RELATIVE)
+    return BatchScene(f"batch_{n_cloths}", side.copy(), side.copy(), X, np.zeros_like(X),
+                      springs, nspr.astype(np.int32), pinned, p, steps)

@@ -69,7 +69,8 @@ typedef struct {
     int32_t tol_mode;   /* 0: stop when r.r <= tol^2 (IE:238); 1: when r.r <= tol^2 * r0.r0
                            (BW:292); 2: no test, run max_it iterations */
     int32_t warm_start; /* 1: CG starts from the previous step's dv (IE:691, BW:788) */
-    int32_t dot_mode;   /* 0: sequential dots (IE:219-225); 1: tile-tree order of the HIP kernels */
+    int32_t dot_mode;   /* 0: sequential dots (IE:219-225); 1: tile-tree order of the
+                           single-cloth HIP kernels; 2: order of the batched kernels */
     int32_t nx, ny, tx, ty; /* grid and tile shape (dot_mode 1) */
     int32_t flags;      /* bit0: clamp y<0 to 0 after the update (IE:695-697) */
     int32_t pad;
@@ -556,6 +557,7 @@ EXPORT void orc_matvec(const orc_sys* s, const real* Cp, double mass, const real
 /* Dot products                                                              */
 /* ------------------------------------------------------------------------ */
 /* mode 0: sequential, sum += dot(a_i,b_i) (IE:219-225).
+ * mode 2: the batched kernels' order (tx = threads per cloth), below.
  * mode 1: the fixed reduction tree of the HIP kernels.  The grid is cut into
  * tx*ty tiles (tiles row-major, threads row-major in a tile, tx*ty a multiple
  * of 64).  Per 64-lane wave: for off = 32,16,..,1 lane l += lane l+off; the
@@ -574,6 +576,23 @@ static real dot_vec(const orc_sys* s, const orc_params* prm, const real* a, cons
         real sum = 0;
         for (int i = 0; i < N; i++) sum += dot3(a + 3 * i, b + 3 * i);
         return sum;
+    }
+    if (prm->dot_mode == 2) {
+        /* batched kernels: T = prm->tx threads per cloth; thread t sums
+         * vertices t, t+T, ... in order; 64-lane wave trees; waves in order */
+        int T = prm->tx, nw = T / 64;
+        real lanes[64];
+        real total = 0;
+        for (int w = 0; w < nw; w++) {
+            for (int l = 0; l < 64; l++) {
+                real s = 0;
+                for (int v = w * 64 + l; v < N; v += T) s += dot3(a + 3 * v, b + 3 * v);
+                lanes[l] = s;
+            }
+            real ws = wave_tree(lanes);
+            total = (w == 0) ? ws : total + ws;
+        }
+        return total;
     }
     int nx = prm->nx, ny = prm->ny, tx = prm->tx, ty = prm->ty;
     int ntx = (nx + tx - 1) / tx, nty = (ny + ty - 1) / ty;
