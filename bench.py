@@ -85,6 +85,66 @@ def cpu_baseline(sc, budget_s: float = 12.0) -> dict:
     }
 
 
+def bench_batch(steps: int, warmup: int, n_cloths: int = 4096, plain_steps: int = 1) -> dict:
+    """Batched forward step: n_cloths cloths with sides uniform in [16,128],
+    ks log-uniform [20,200], kd uniform [0.1,0.5], 2-8 random top-edge pins,
+    CG <= 40 iterations with per-cloth relative tolerance 1e-4, dt 1/60, one
+    persistent launch per step replayed from a HIP graph.  The plain form
+    steps the same cloths one cloth per launch sequence (oc_cloth_step)."""
+    from opencloth_amd import scenes
+    from opencloth_amd.batch import ClothBatch
+    from opencloth_amd.cloth import ImplicitCloth
+    from opencloth_amd.hiptime import Event
+
+    t0 = time.perf_counter()
+    sc = scenes.batch_scene(n_cloths=n_cloths)
+    t1 = time.perf_counter()
+    b = ClothBatch.from_scene(sc)
+    t2 = time.perf_counter()
+    sched = b.autotune()
+    b.step(warmup)
+    b.synchronize()
+    e0, e1 = Event(), Event()
+    e0.record()
+    b.step(steps)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1)
+    it, st = b.status()
+    b.close()
+    N = int(sc.X.shape[0])
+
+    plain = [ImplicitCloth.from_scene(sc.cloth(c)) for c in range(sc.n_cloths)]
+    for cl in plain:
+        cl.step(1)
+    for cl in plain:
+        cl.synchronize()
+    p0, p1 = Event(), Event()
+    p0.record()
+    for _ in range(plain_steps):
+        for cl in plain:
+            cl.step(1)
+    p1.record()
+    p1.synchronize()
+    ms_plain = p0.elapsed_ms(p1) / plain_steps
+    for cl in plain:
+        cl.close()
+    return {
+        "workload": f"{sc.n_cloths} cloths, sides U[16,128], {N} vertices, {len(sc.springs)} springs, "
+                    "CG <= 40 it, rel tol 1e-4, dt 1/60",
+        "vertex_steps_per_s": N * steps / (ms * 1e-3),
+        "ms_per_step": ms / steps,
+        "steps": steps,
+        "schedule": {"block_threads": sched[0], "vertices_per_thread": sched[1]},
+        "cg_iters_mean": float(it.mean()), "cg_iters_max": int(it.max()),
+        "status_nonzero": int((st != 0).sum()),
+        "plain_ms_per_step": ms_plain,
+        "plain_vertex_steps_per_s": N / (ms_plain * 1e-3),
+        "speedup_vs_plain": ms_plain / (ms / steps),
+        "setup_s": {"scene": t1 - t0, "create": t2 - t1},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +152,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="large", choices=["large", "plain"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
+    ap.add_argument("--batch-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -158,6 +220,8 @@ def main():
             "avg_launch_us": spmv_avg_s * 1e6,
         },
     }
+    if not args.no_batch:
+        out["batch"] = bench_batch(args.batch_steps, 2)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(out), flush=True)

@@ -143,6 +143,25 @@ int oc_cloth_synchronize(oc_cloth cloth);
 /* Status of the last solve: CG iterations and OC_SOLVE_* bits; synchronises. */
 int oc_cloth_last_status(oc_cloth cloth, int32_t* iters, int32_t* status);
 
+/* ---- differentiable rollouts (no counterpart in the reference) ---------- */
+/* Per-spring stiffness Ks (n_springs floats, the spring list's order); the
+ * reference fixes Ks per spring type (IE:310-312). */
+int oc_cloth_set_spring_stiffness(oc_cloth cloth, const float* ks);
+int oc_cloth_get_spring_stiffness(oc_cloth cloth, float* ks);
+/* Run nsteps implicit steps from the current state with dv reset to zero,
+ * keeping a device checkpoint (x, v, dv) every checkpoint_every steps for
+ * oc_cloth_rollout_backward.  The state ends at step nsteps. */
+int oc_cloth_rollout_forward(oc_cloth cloth, int32_t nsteps, int32_t checkpoint_every);
+/* Hand-written adjoint of the last rollout_forward: given dL/dx and dL/dv at
+ * its final state (host AoS; grad_v may be NULL for zero), return dL/dks
+ * (n_springs floats), dL/dx0 and dL/dv0 (host AoS; any output may be NULL).
+ * Each segment between checkpoints is recomputed forward, then every step's
+ * adjoint solves the same symmetric system (filtered CG) and gathers the
+ * derivatives of the spring forces and Jacobians per vertex.  The state is
+ * restored to the rollout's final state; synchronises. */
+int oc_cloth_rollout_backward(oc_cloth cloth, const float* grad_x, const float* grad_v,
+                              float* grad_ks, float* grad_x0, float* grad_v0);
+
 /* Measurement hook (no counterpart in the reference, whose only timing is a
  * frame counter, IE:562-584): runs nsteps like oc_cloth_step with a HIP
  * event after every kernel launch and returns, per kernel class
@@ -201,8 +220,19 @@ int oc_batch_get_state(oc_batch batch, float* x, float* v);
 int oc_batch_step(oc_batch batch, int32_t nsteps);
 /* nsteps steps launched directly (no graph). */
 int oc_batch_step_eager(oc_batch batch, int32_t nsteps);
-/* CG iterations and OC_SOLVE_* bits of every cloth's last step; synchronises. */
+/* CG iterations and OC_SOLVE_* bits of every cloth's last step (or, after
+ * oc_batch_rollout_backward, of its last adjoint solve); synchronises. */
 int oc_batch_get_status(oc_batch batch, int32_t* iters, int32_t* status);
+/* Per-spring stiffness of all cloths (concatenated spring lists). */
+int oc_batch_set_spring_stiffness(oc_batch batch, const float* ks);
+int oc_batch_get_spring_stiffness(oc_batch batch, float* ks);
+/* Batched differentiable rollout: as oc_cloth_rollout_forward/backward for
+ * every cloth at once; each step's adjoint is one persistent launch that
+ * re-assembles, solves the adjoint system per cloth (masked CG) and gathers
+ * the derivatives.  Arrays are all cloths' vertices/springs concatenated. */
+int oc_batch_rollout_forward(oc_batch batch, int32_t nsteps, int32_t checkpoint_every);
+int oc_batch_rollout_backward(oc_batch batch, const float* grad_x, const float* grad_v,
+                              float* grad_ks, float* grad_x0, float* grad_v0);
 int oc_batch_synchronize(oc_batch batch);
 
 #ifdef __cplusplus

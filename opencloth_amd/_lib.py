@@ -46,6 +46,10 @@ SIGNATURES = {
     "oc_cloth_synchronize": (C.c_int, [_P]),
     "oc_cloth_last_status": (C.c_int, [_P, _P, _P]),
     "oc_cloth_profile_steps": (C.c_int, [_P, _I, _P, _P]),
+    "oc_cloth_set_spring_stiffness": (C.c_int, [_P, _P]),
+    "oc_cloth_get_spring_stiffness": (C.c_int, [_P, _P]),
+    "oc_cloth_rollout_forward": (C.c_int, [_P, _I, _I]),
+    "oc_cloth_rollout_backward": (C.c_int, [_P, _P, _P, _P, _P, _P]),
     "oc_cloth_num_pairs": (C.c_int, [_P, _P]),
     "oc_cloth_get_pairs": (C.c_int, [_P, _P, _P]),
     "oc_cloth_get_system": (C.c_int, [_P, _P, _P]),
@@ -63,6 +67,10 @@ SIGNATURES = {
     "oc_batch_step_eager": (C.c_int, [_P, _I]),
     "oc_batch_get_status": (C.c_int, [_P, _P, _P]),
     "oc_batch_synchronize": (C.c_int, [_P]),
+    "oc_batch_set_spring_stiffness": (C.c_int, [_P, _P]),
+    "oc_batch_get_spring_stiffness": (C.c_int, [_P, _P]),
+    "oc_batch_rollout_forward": (C.c_int, [_P, _I, _I]),
+    "oc_batch_rollout_backward": (C.c_int, [_P, _P, _P, _P, _P, _P]),
 }
 
 

@@ -30,6 +30,7 @@ class ClothBatch:
         check(lib().oc_batch_create(self.n, ptr(self.nx), ptr(self.ny), ptr(X), ptr(V), ptr(springs),
                                     ptr(ns), ptr(pins), C.byref(self._p), C.byref(h)))
         self.h = h
+        self.n_springs_total = int(ns.astype(np.int64).sum())
 
     @classmethod
     def from_scene(cls, sc: BatchScene) -> "ClothBatch":
@@ -96,3 +97,26 @@ class ClothBatch:
 
     def synchronize(self):
         check(lib().oc_batch_synchronize(self.h))
+
+    # -- differentiable rollouts ---------------------------------------------
+    def set_spring_stiffness(self, ks):
+        k = np.ascontiguousarray(ks, np.float32)
+        check(lib().oc_batch_set_spring_stiffness(self.h, ptr(k)))
+
+    def spring_stiffness(self, n_springs: int) -> np.ndarray:
+        k = np.zeros(n_springs, np.float32)
+        check(lib().oc_batch_get_spring_stiffness(self.h, ptr(k)))
+        return k
+
+    def rollout_forward(self, nsteps: int, checkpoint_every: int = 10):
+        check(lib().oc_batch_rollout_forward(self.h, int(nsteps), int(checkpoint_every)))
+
+    def rollout_backward(self, grad_x, grad_v=None, n_springs: int | None = None):
+        """(dL/dks, dL/dx0, dL/dv0) of the last rollout_forward given dL/dx_T."""
+        gx = np.ascontiguousarray(grad_x, np.float32).reshape(self.N, 3)
+        gv = None if grad_v is None else np.ascontiguousarray(grad_v, np.float32).reshape(self.N, 3)
+        gks = np.zeros(n_springs if n_springs is not None else self.n_springs_total, np.float32)
+        gx0 = np.zeros((self.N, 3), np.float32)
+        gv0 = np.zeros((self.N, 3), np.float32)
+        check(lib().oc_batch_rollout_backward(self.h, ptr(gx), ptr(gv), ptr(gks), ptr(gx0), ptr(gv0)))
+        return gks, gx0, gv0

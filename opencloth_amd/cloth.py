@@ -157,6 +157,32 @@ class ImplicitCloth:
         check(lib().oc_cloth_last_status(self.h, C.byref(it), C.byref(st)))
         return it.value, st.value
 
+    # -- differentiable rollouts ---------------------------------------------
+    def set_spring_stiffness(self, ks):
+        k = np.ascontiguousarray(ks, np.float32)
+        if k.shape != (self.S,):
+            raise ValueError(f"expected {self.S} stiffness values")
+        check(lib().oc_cloth_set_spring_stiffness(self.h, ptr(k)))
+
+    def spring_stiffness(self) -> np.ndarray:
+        k = np.zeros(self.S, np.float32)
+        check(lib().oc_cloth_get_spring_stiffness(self.h, ptr(k)))
+        return k
+
+    def rollout_forward(self, nsteps: int, checkpoint_every: int = 16):
+        """nsteps steps from the current state (dv reset), checkpointed."""
+        check(lib().oc_cloth_rollout_forward(self.h, int(nsteps), int(checkpoint_every)))
+
+    def rollout_backward(self, grad_x, grad_v=None):
+        """(dL/dks, dL/dx0, dL/dv0) of the last rollout_forward given dL/dx_T."""
+        gx = _vec(grad_x, self.N)
+        gv = None if grad_v is None else _vec(grad_v, self.N)
+        gks = np.zeros(self.S, np.float32)
+        gx0 = np.zeros((self.N, 3), np.float32)
+        gv0 = np.zeros((self.N, 3), np.float32)
+        check(lib().oc_cloth_rollout_backward(self.h, ptr(gx), ptr(gv), ptr(gks), ptr(gx0), ptr(gv0)))
+        return gks, gx0, gv0
+
     # -- introspection -------------------------------------------------------
     def pairs(self):
         a = np.zeros(self.P, np.int32); b = np.zeros(self.P, np.int32)

@@ -118,8 +118,63 @@ __device__ inline float spmv_vertex(const DevBatch& c, const StepScalars& s, int
     return dot3(di, q);
 }
 
-template <int NT>
-__global__ __launch_bounds__(NT) void k_batch_step(DevBatch c, StepScalars s) {
+// Adjoint start for vertex i: gvt = gv' + h gx', r = d = S gvt (the solution
+// buffer, c.dv, was zeroed by the cold assembly); returns r.r.
+__device__ inline float adjoint_start_vertex(const DevBatch& c, const StepScalars& s,
+                                             const BatchAdjBufs& a, int i) {
+    const int N = c.Ntot;
+    const V3 gx = ld3(a.gx_out, N, i), gv = ld3(a.gv_out, N, i);
+    const V3 g = {gv.x + s.h * gx.x, gv.y + s.h * gx.y, gv.z + s.h * gx.z};
+    st3(a.gvt, N, i, g);
+    const V3 r = c.pin[i] ? V3{0.0f, 0.0f, 0.0f} : g;
+    st3(c.r, N, i, r);
+    st3(c.d, N, i, r);
+    return dot3(r, r);
+}
+
+// Adjoint gather for vertex i (same as the single-cloth k_adj_gather):
+// c.x, c.v the pre-step state, c.dv the adjoint solve mu, a.dvsol the step's
+// solution.
+__device__ inline void adjoint_gather_vertex(const DevBatch& c, const StepScalars& s,
+                                             const BatchAdjBufs& a, int i, int nx) {
+    const int N = c.Ntot;
+    const V3 xi = ld3(c.x, N, i), vi = ld3(c.v, N, i), mi = ld3(c.dv, N, i),
+             pi = ld3(a.dvsol, N, i);
+    const V3 wi = add3(vi, pi);
+    V3 gx = ld3(a.gx_out, N, i), gv = ld3(a.gvt, N, i);
+    for (int k = 0; k < c.wa; k++) {
+        const int sp = c.ainc_s[(size_t)k * N + i];
+        if (sp < 0) break;
+        const uint16_t m = c.ainc_m[(size_t)k * N + i];
+        const int j = i + slot_dj(m) * nx + slot_di(m);
+        const V3 xj = ld3(c.x, N, j), vj = ld3(c.v, N, j), mj = ld3(c.dv, N, j),
+                 pj = ld3(a.dvsol, N, j);
+        const V3 wj = add3(vj, pj);
+        const bool second = (m >> 8) & 1u;
+        float ge[3], gvv[3], gk;
+        if (second)
+            spring_adjoint(xj, xi, vj, vi, mj, mi, wj, wi, pj, pi, c.L0[sp], c.ks[sp], c.kd[sp], s.h,
+                           s.hh, ge, gvv, gk);
+        else
+            spring_adjoint(xi, xj, vi, vj, mi, mj, wi, wj, pi, pj, c.L0[sp], c.ks[sp], c.kd[sp], s.h,
+                           s.hh, ge, gvv, gk);
+        if (second) {
+            gx = {gx.x - ge[0], gx.y - ge[1], gx.z - ge[2]};
+            gv = {gv.x - gvv[0], gv.y - gvv[1], gv.z - gvv[2]};
+        } else {
+            gx = {gx.x + ge[0], gx.y + ge[1], gx.z + ge[2]};
+            gv = {gv.x + gvv[0], gv.y + gvv[1], gv.z + gvv[2]};
+            a.gks[sp] = a.gks[sp] + gk;
+        }
+    }
+    st3(a.gx_in, N, i, gx);
+    st3(a.gv_in, N, i, gv);
+}
+
+// ADJ = false: forward step.  ADJ = true: one step's adjoint at the state in
+// c.x/c.v (assembly without warm start, CG for mu in c.dv, gather).
+template <int NT, bool ADJ>
+__global__ __launch_bounds__(NT) void k_batch_step(DevBatch c, StepScalars s, BatchAdjBufs adj) {
     constexpr int NW = NT / 64;
     __shared__ float ws[NW];
     __shared__ float sd[NW], sd_old[NW], sd0[NW], ssig[NW];
@@ -146,7 +201,13 @@ __global__ __launch_bounds__(NT) void k_batch_step(DevBatch c, StepScalars s) {
 
         // ---- assembly and CG start
         float t = 0.0f;
-        for (int vv = lt; vv < Ni; vv += T) t += assemble_vertex(c, s, voff + vv, nx);
+        for (int vv = lt; vv < Ni; vv += T) {
+            const float rr = assemble_vertex(c, s, voff + vv, nx);
+            if constexpr (ADJ)
+                t += adjoint_start_vertex(c, s, adj, voff + vv);
+            else
+                t += rr;
+        }
         float tot = seg_sum<NW>(t, T, ws);
         if (act && lt == 0) {
             sd[g] = tot;
@@ -235,19 +296,23 @@ __global__ __launch_bounds__(NT) void k_batch_step(DevBatch c, StepScalars s) {
             sst[g] = st;
         }
         __syncthreads();
-        bool bad = false;
-        for (int vv = lt; vv < Ni; vv += T) {
-            const int i = voff + vv;
-            V3 v = ld3(c.v, N, i), dv = ld3(c.dv, N, i), x = ld3(c.x, N, i);
-            v = add3(v, dv);
-            x = {x.x + s.h * v.x, x.y + s.h * v.y, x.z + s.h * v.z};
-            if ((s.flags & OC_FLAG_GROUND_CLAMP) && x.y < 0.0f) x.y = 0.0f;
-            st3(c.v, N, i, v);
-            st3(c.x, N, i, x);
-            bad = bad || !(isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) &&
-                           isfinite(v.y) && isfinite(v.z));
+        if constexpr (ADJ) {
+            for (int vv = lt; vv < Ni; vv += T) adjoint_gather_vertex(c, s, adj, voff + vv, nx);
+        } else {
+            bool bad = false;
+            for (int vv = lt; vv < Ni; vv += T) {
+                const int i = voff + vv;
+                V3 v = ld3(c.v, N, i), dv = ld3(c.dv, N, i), x = ld3(c.x, N, i);
+                v = add3(v, dv);
+                x = {x.x + s.h * v.x, x.y + s.h * v.y, x.z + s.h * v.z};
+                if ((s.flags & OC_FLAG_GROUND_CLAMP) && x.y < 0.0f) x.y = 0.0f;
+                st3(c.v, N, i, v);
+                st3(c.x, N, i, x);
+                bad = bad || !(isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) &&
+                               isfinite(v.y) && isfinite(v.z));
+            }
+            if (bad) atomicOr(&sst[g], OC_SOLVE_NONFINITE);
         }
-        if (bad) atomicOr(&sst[g], OC_SOLVE_NONFINITE);
         __syncthreads();
         if (act && lt == 0) {
             c.inst_iters[ins] = sit[g];
@@ -268,22 +333,33 @@ hipError_t launch_batch_reset(const DevBatch& c, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_batch_step(const DevBatch& c, const StepScalars& s, int block_threads, int grid,
-                             hipStream_t st) {
+template <bool ADJ>
+static hipError_t launch_batch(const DevBatch& c, const StepScalars& s, const BatchAdjBufs& a,
+                               int block_threads, int grid, hipStream_t st) {
     switch (block_threads) {
         case 256:
-            hipLaunchKernelGGL((k_batch_step<256>), dim3(grid), dim3(256), 0, st, c, s);
+            hipLaunchKernelGGL((k_batch_step<256, ADJ>), dim3(grid), dim3(256), 0, st, c, s, a);
             break;
         case 512:
-            hipLaunchKernelGGL((k_batch_step<512>), dim3(grid), dim3(512), 0, st, c, s);
+            hipLaunchKernelGGL((k_batch_step<512, ADJ>), dim3(grid), dim3(512), 0, st, c, s, a);
             break;
         case 1024:
-            hipLaunchKernelGGL((k_batch_step<1024>), dim3(grid), dim3(1024), 0, st, c, s);
+            hipLaunchKernelGGL((k_batch_step<1024, ADJ>), dim3(grid), dim3(1024), 0, st, c, s, a);
             break;
         default:
             return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_batch_step(const DevBatch& c, const StepScalars& s, int block_threads, int grid,
+                             hipStream_t st) {
+    return launch_batch<false>(c, s, BatchAdjBufs{}, block_threads, grid, st);
+}
+
+hipError_t launch_batch_adjoint(const DevBatch& c, const StepScalars& s, const BatchAdjBufs& a,
+                                int block_threads, int grid, hipStream_t st) {
+    return launch_batch<true>(c, s, a, block_threads, grid, st);
 }
 
 }  // namespace oc

@@ -48,6 +48,11 @@ struct oc_cloth_s {
     int* ctl_host = nullptr;      // pinned copy of ctl
     oc::DevCloth dc{};
     bool assembled = false;
+    // differentiable rollouts: checkpoints, one segment of states, adjoint vectors
+    void* adj_arena = nullptr;
+    size_t adj_bytes = 0;
+    int roll_steps = 0, roll_every = 0;
+    bool roll_valid = false;
 };
 
 namespace {
@@ -272,6 +277,7 @@ int oc_cloth_destroy(oc_cloth c) {
     if (c->arena) (void)hipFree(c->arena);
     if (c->scratch) (void)hipFree(c->scratch);
     if (c->ctl_host) (void)hipHostFree(c->ctl_host);
+    if (c->adj_arena) (void)hipFree(c->adj_arena);
     delete c;
     return OC_OK;
 }
@@ -471,6 +477,190 @@ int oc_cloth_get_system(oc_cloth c, float* blocks, float* rhs) {
                     planes[((size_t)t.pair_plane[p] * 6 + q) * N + (size_t)t.pair_a[p]];
     }
     if (rhs) return download_vec(c, c->dc.b, rhs);
+    return OC_OK;
+}
+
+int oc_cloth_set_spring_stiffness(oc_cloth c, const float* ks) {
+    if (!c || !ks) return fail(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    OC_HIP(hipMemcpyAsync((void*)c->dc.ks, ks, sizeof(float) * c->topo.S, hipMemcpyHostToDevice,
+                          c->stream));
+    OC_HIP(hipStreamSynchronize(c->stream));
+    c->roll_valid = false;
+    return OC_OK;
+}
+
+int oc_cloth_get_spring_stiffness(oc_cloth c, float* ks) {
+    if (!c || !ks) return fail(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    OC_HIP(hipMemcpyAsync(ks, c->dc.ks, sizeof(float) * c->topo.S, hipMemcpyDeviceToHost,
+                          c->stream));
+    OC_HIP(hipStreamSynchronize(c->stream));
+    return OC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Views into the rollout arena: checkpoints [(nseg+1) x (x,v,dv)], one
+// segment of pre-step states and solutions [every x (x,v,dv)], 11 adjoint
+// vectors and the per-spring gradient.
+struct RolloutViews {
+    float* ckpt;
+    float* seg;
+    float* vec[11];
+    float* gks;
+};
+
+RolloutViews rollout_views(oc_cloth_s* c, int nseg, int every) {
+    const size_t v3 = 3 * (size_t)c->dc.N;
+    RolloutViews r;
+    float* p = (float*)c->adj_arena;
+    r.ckpt = p;
+    p += (size_t)(nseg + 1) * 3 * v3;
+    r.seg = p;
+    p += (size_t)every * 3 * v3;
+    for (int k = 0; k < 11; k++) {
+        r.vec[k] = p;
+        p += v3;
+    }
+    r.gks = p;
+    return r;
+}
+
+int save_state(oc_cloth_s* c, float* dst) {
+    const size_t b = sizeof(float) * 3 * (size_t)c->dc.N;
+    OC_HIP(hipMemcpyAsync(dst, c->dc.x, b, hipMemcpyDeviceToDevice, c->stream));
+    OC_HIP(hipMemcpyAsync((char*)dst + b, c->dc.v, b, hipMemcpyDeviceToDevice, c->stream));
+    OC_HIP(hipMemcpyAsync((char*)dst + 2 * b, c->dc.dv, b, hipMemcpyDeviceToDevice, c->stream));
+    return OC_OK;
+}
+
+int load_state(oc_cloth_s* c, const float* src) {
+    const size_t b = sizeof(float) * 3 * (size_t)c->dc.N;
+    OC_HIP(hipMemcpyAsync(c->dc.x, src, b, hipMemcpyDeviceToDevice, c->stream));
+    OC_HIP(hipMemcpyAsync(c->dc.v, (const char*)src + b, b, hipMemcpyDeviceToDevice, c->stream));
+    OC_HIP(hipMemcpyAsync(c->dc.dv, (const char*)src + 2 * b, b, hipMemcpyDeviceToDevice, c->stream));
+    return OC_OK;
+}
+
+int one_step(oc_cloth_s* c, const oc::StepScalars& s) {
+    OC_HIP(oc::launch_assemble(c->dc, s, c->stream));
+    OC_HIP(oc::launch_cg(c->dc, s, c->stream));
+    OC_HIP(oc::launch_update(c->dc, s, c->stream));
+    return OC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oc_cloth_rollout_forward(oc_cloth c, int32_t nsteps, int32_t every) {
+    if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
+    if (nsteps < 1 || every < 1) return fail(OC_ERR_INVALID_ARGUMENT, "nsteps and checkpoint_every must be >= 1");
+    every = std::min(every, nsteps);
+    const int nseg = (nsteps + every - 1) / every;
+    const size_t v3 = 3 * (size_t)c->dc.N;
+    const size_t need = sizeof(float) * ((size_t)(nseg + 1) * 3 * v3 + (size_t)every * 3 * v3 +
+                                         11 * v3 + (size_t)std::max(c->topo.S, 1));
+    if (need > c->adj_bytes) {
+        if (c->adj_arena) {
+            OC_HIP(hipStreamSynchronize(c->stream));
+            OC_HIP(hipFree(c->adj_arena));
+            c->adj_arena = nullptr;
+            c->adj_bytes = 0;
+        }
+        if (hipMalloc(&c->adj_arena, need) != hipSuccess)
+            return fail(OC_ERR_OUT_OF_MEMORY, "hipMalloc of rollout checkpoints failed");
+        c->adj_bytes = need;
+    }
+    RolloutViews rv = rollout_views(c, nseg, every);
+    const oc::StepScalars s = scalars(c, c->prm.dt);
+    OC_HIP(hipMemsetAsync(c->dc.dv, 0, sizeof(float) * v3, c->stream));
+    int rc;
+    for (int n = 0; n < nsteps; n++) {
+        if (n % every == 0 && (rc = save_state(c, rv.ckpt + (size_t)(n / every) * 3 * v3)) != OC_OK)
+            return rc;
+        if ((rc = one_step(c, s)) != OC_OK) return rc;
+    }
+    if ((rc = save_state(c, rv.ckpt + (size_t)nseg * 3 * v3)) != OC_OK) return rc;
+    c->roll_steps = nsteps;
+    c->roll_every = every;
+    c->roll_valid = true;
+    c->assembled = false;
+    return OC_OK;
+}
+
+int oc_cloth_rollout_backward(oc_cloth c, const float* grad_x, const float* grad_v, float* grad_ks,
+                              float* grad_x0, float* grad_v0) {
+    if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
+    if (!grad_x) return fail(OC_ERR_INVALID_ARGUMENT, "grad_x is NULL");
+    if (!c->roll_valid) return fail(OC_ERR_INVALID_ARGUMENT, "no rollout_forward to differentiate");
+    const int nsteps = c->roll_steps, every = c->roll_every;
+    const int nseg = (nsteps + every - 1) / every;
+    const size_t v3 = 3 * (size_t)c->dc.N;
+    RolloutViews rv = rollout_views(c, nseg, every);
+    float* mu = rv.vec[0];
+    float* gx[2] = {rv.vec[7], rv.vec[8]};
+    float* gv[2] = {rv.vec[9], rv.vec[10]};
+    int rc;
+    if ((rc = upload_vec(c, gx[0], grad_x)) != OC_OK) return rc;
+    if ((rc = upload_vec(c, gv[0], grad_v)) != OC_OK) return rc;
+    OC_HIP(hipMemsetAsync(rv.gks, 0, sizeof(float) * std::max(c->topo.S, 1), c->stream));
+    const oc::StepScalars s = scalars(c, c->prm.dt);
+    oc::StepScalars s_cold = s;
+    s_cold.warm = 0;
+    int cur = 0;
+    for (int seg = nseg - 1; seg >= 0; seg--) {
+        const int n0 = seg * every, n1 = std::min(n0 + every, nsteps);
+        if ((rc = load_state(c, rv.ckpt + (size_t)seg * 3 * v3)) != OC_OK) return rc;
+        for (int n = n0; n < n1; n++) {
+            float* slot = rv.seg + (size_t)(n - n0) * 3 * v3;
+            const size_t b = sizeof(float) * v3;
+            OC_HIP(hipMemcpyAsync(slot, c->dc.x, b, hipMemcpyDeviceToDevice, c->stream));
+            OC_HIP(hipMemcpyAsync(slot + v3, c->dc.v, b, hipMemcpyDeviceToDevice, c->stream));
+            if ((rc = one_step(c, s)) != OC_OK) return rc;
+            OC_HIP(hipMemcpyAsync(slot + 2 * v3, c->dc.dv, b, hipMemcpyDeviceToDevice, c->stream));
+        }
+        for (int n = n1 - 1; n >= n0; n--) {
+            float* slot = rv.seg + (size_t)(n - n0) * 3 * v3;
+            oc::DevCloth ca = c->dc;  // adjoint solve view: own CG vectors
+            ca.x = slot;
+            ca.v = slot + v3;
+            ca.dv = mu;
+            ca.r = rv.vec[1];
+            ca.d0 = rv.vec[2];
+            ca.d1 = rv.vec[3];
+            ca.q = rv.vec[4];
+            ca.b = rv.vec[5];
+            oc::AdjointBufs ab;
+            ab.gx_out = gx[cur];
+            ab.gv_out = gv[cur];
+            ab.gvt = rv.vec[6];
+            ab.mu = mu;
+            ab.gx_in = gx[1 - cur];
+            ab.gv_in = gv[1 - cur];
+            ab.gks = rv.gks;
+            OC_HIP(oc::launch_assemble(ca, s_cold, c->stream));
+            OC_HIP(oc::launch_adj_init(ca, s, ab, c->stream));
+            OC_HIP(oc::launch_cg(ca, s, c->stream));
+            oc::DevCloth cg = c->dc;  // pre-step state and the step's solution
+            cg.x = slot;
+            cg.v = slot + v3;
+            cg.dv = slot + 2 * v3;
+            OC_HIP(oc::launch_adj_gather(cg, s, ab, c->stream));
+            cur = 1 - cur;
+        }
+    }
+    if ((rc = load_state(c, rv.ckpt + (size_t)nseg * 3 * v3)) != OC_OK) return rc;
+    if (grad_ks) {
+        OC_HIP(hipMemcpyAsync(grad_ks, rv.gks, sizeof(float) * c->topo.S, hipMemcpyDeviceToHost,
+                              c->stream));
+        OC_HIP(hipStreamSynchronize(c->stream));
+    }
+    if (grad_x0 && (rc = download_vec(c, gx[cur], grad_x0)) != OC_OK) return rc;
+    if (grad_v0 && (rc = download_vec(c, gv[cur], grad_v0)) != OC_OK) return rc;
+    OC_HIP(hipStreamSynchronize(c->stream));
+    c->assembled = false;
     return OC_OK;
 }
 

@@ -53,6 +53,11 @@ struct oc_batch_s {
     std::vector<int> T_of;      // threads per cloth under the schedule
     int grid = 0;
     oc::DevBatch dc{};
+    // differentiable rollouts
+    void* adj_arena = nullptr;
+    size_t adj_bytes = 0;
+    int roll_steps = 0, roll_every = 0;
+    bool roll_valid = false;
 };
 
 namespace {
@@ -372,6 +377,7 @@ int oc_batch_destroy(oc_batch b) {
     if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
     if (b->sched_buf) (void)hipFree(b->sched_buf);
     if (b->arena) (void)hipFree(b->arena);
+    if (b->adj_arena) (void)hipFree(b->adj_arena);
     delete b;
     return OC_OK;
 }
@@ -539,6 +545,196 @@ int oc_batch_autotune(oc_batch b, oc_batch_schedule* best) {
     if (rc != OC_OK) return rc;
     if (rc2 != OC_OK) return rc2;
     if (best) *best = win;
+    return OC_OK;
+}
+
+int oc_batch_set_spring_stiffness(oc_batch b, const float* ks) {
+    if (!b || !ks) return set_error(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    OC_HIP(hipStreamSynchronize(b->stream));
+    OC_HIP(hipMemcpy((void*)b->dc.ks, ks, sizeof(float) * b->Stot, hipMemcpyHostToDevice));
+    b->roll_valid = false;
+    return OC_OK;
+}
+
+int oc_batch_get_spring_stiffness(oc_batch b, float* ks) {
+    if (!b || !ks) return set_error(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    OC_HIP(hipStreamSynchronize(b->stream));
+    OC_HIP(hipMemcpy(ks, b->dc.ks, sizeof(float) * b->Stot, hipMemcpyDeviceToHost));
+    return OC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Rollout arena: checkpoints [(nseg+1) x (x,v,dv)], one segment of pre-step
+// states and solutions [every x (x,v,dv)], mu, gvt, gx[2], gv[2], gks.
+struct BatchRollout {
+    float* ckpt;
+    float* seg;
+    float* mu;
+    float* gvt;
+    float* gx[2];
+    float* gv[2];
+    float* gks;
+};
+
+BatchRollout batch_rollout_views(oc_batch_s* b, int nseg, int every) {
+    const size_t v3 = 3 * (size_t)b->Ntot;
+    BatchRollout r;
+    float* p = (float*)b->adj_arena;
+    r.ckpt = p;
+    p += (size_t)(nseg + 1) * 3 * v3;
+    r.seg = p;
+    p += (size_t)every * 3 * v3;
+    r.mu = p;
+    p += v3;
+    r.gvt = p;
+    p += v3;
+    for (int k = 0; k < 2; k++) {
+        r.gx[k] = p;
+        p += v3;
+        r.gv[k] = p;
+        p += v3;
+    }
+    r.gks = p;
+    return r;
+}
+
+int batch_copy3(oc_batch_s* b, float* dst, const float* x, const float* v, const float* dv) {
+    const size_t n = sizeof(float) * 3 * (size_t)b->Ntot;
+    OC_HIP(hipMemcpyAsync(dst, x, n, hipMemcpyDeviceToDevice, b->stream));
+    OC_HIP(hipMemcpyAsync((char*)dst + n, v, n, hipMemcpyDeviceToDevice, b->stream));
+    OC_HIP(hipMemcpyAsync((char*)dst + 2 * n, dv, n, hipMemcpyDeviceToDevice, b->stream));
+    return OC_OK;
+}
+
+int batch_restore(oc_batch_s* b, const float* src) {
+    const size_t n = sizeof(float) * 3 * (size_t)b->Ntot;
+    OC_HIP(hipMemcpyAsync(b->dc.x, src, n, hipMemcpyDeviceToDevice, b->stream));
+    OC_HIP(hipMemcpyAsync(b->dc.v, (const char*)src + n, n, hipMemcpyDeviceToDevice, b->stream));
+    OC_HIP(hipMemcpyAsync(b->dc.dv, (const char*)src + 2 * n, n, hipMemcpyDeviceToDevice, b->stream));
+    return OC_OK;
+}
+
+int batch_upload_aos(oc_batch_s* b, float* dst, const float* aos) {
+    const size_t N = b->Ntot;
+    std::vector<float> t(3 * N, 0.0f);
+    if (aos)
+        for (size_t i = 0; i < N; i++)
+            for (int k = 0; k < 3; k++) t[k * N + i] = aos[3 * i + k];
+    OC_HIP(hipMemcpyAsync(dst, t.data(), t.size() * 4, hipMemcpyHostToDevice, b->stream));
+    OC_HIP(hipStreamSynchronize(b->stream));
+    return OC_OK;
+}
+
+int batch_download_aos(oc_batch_s* b, const float* src, float* aos) {
+    const size_t N = b->Ntot;
+    std::vector<float> t(3 * N);
+    OC_HIP(hipMemcpyAsync(t.data(), src, t.size() * 4, hipMemcpyDeviceToHost, b->stream));
+    OC_HIP(hipStreamSynchronize(b->stream));
+    for (size_t i = 0; i < N; i++)
+        for (int k = 0; k < 3; k++) aos[3 * i + k] = t[k * N + i];
+    return OC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oc_batch_rollout_forward(oc_batch b, int32_t nsteps, int32_t every) {
+    if (!b) return set_error(OC_ERR_INVALID_ARGUMENT, "batch is NULL");
+    if (nsteps < 1 || every < 1)
+        return set_error(OC_ERR_INVALID_ARGUMENT, "nsteps and checkpoint_every must be >= 1");
+    every = std::min(every, nsteps);
+    const int nseg = (nsteps + every - 1) / every;
+    const size_t v3 = 3 * (size_t)b->Ntot;
+    const size_t need = sizeof(float) * ((size_t)(nseg + 1) * 3 * v3 + (size_t)every * 3 * v3 +
+                                         6 * v3 + (size_t)std::max(b->Stot, 1));
+    if (need > b->adj_bytes) {
+        if (b->adj_arena) {
+            OC_HIP(hipStreamSynchronize(b->stream));
+            OC_HIP(hipFree(b->adj_arena));
+            b->adj_arena = nullptr;
+            b->adj_bytes = 0;
+        }
+        if (hipMalloc(&b->adj_arena, need) != hipSuccess)
+            return set_error(OC_ERR_OUT_OF_MEMORY, "hipMalloc of rollout checkpoints failed");
+        b->adj_bytes = need;
+    }
+    BatchRollout rv = batch_rollout_views(b, nseg, every);
+    int rc = ensure_graph(b);
+    if (rc != OC_OK) return rc;
+    OC_HIP(hipMemsetAsync(b->dc.dv, 0, sizeof(float) * v3, b->stream));
+    for (int n = 0; n < nsteps; n++) {
+        if (n % every == 0 &&
+            (rc = batch_copy3(b, rv.ckpt + (size_t)(n / every) * 3 * v3, b->dc.x, b->dc.v, b->dc.dv)) != OC_OK)
+            return rc;
+        OC_HIP(hipGraphLaunch(b->graph, b->stream));
+    }
+    if ((rc = batch_copy3(b, rv.ckpt + (size_t)nseg * 3 * v3, b->dc.x, b->dc.v, b->dc.dv)) != OC_OK)
+        return rc;
+    b->roll_steps = nsteps;
+    b->roll_every = every;
+    b->roll_valid = true;
+    return OC_OK;
+}
+
+int oc_batch_rollout_backward(oc_batch b, const float* grad_x, const float* grad_v, float* grad_ks,
+                              float* grad_x0, float* grad_v0) {
+    if (!b) return set_error(OC_ERR_INVALID_ARGUMENT, "batch is NULL");
+    if (!grad_x) return set_error(OC_ERR_INVALID_ARGUMENT, "grad_x is NULL");
+    if (!b->roll_valid) return set_error(OC_ERR_INVALID_ARGUMENT, "no rollout_forward to differentiate");
+    const int nsteps = b->roll_steps, every = b->roll_every;
+    const int nseg = (nsteps + every - 1) / every;
+    const size_t v3 = 3 * (size_t)b->Ntot;
+    BatchRollout rv = batch_rollout_views(b, nseg, every);
+    int rc;
+    if ((rc = ensure_graph(b)) != OC_OK) return rc;
+    if ((rc = batch_upload_aos(b, rv.gx[0], grad_x)) != OC_OK) return rc;
+    if ((rc = batch_upload_aos(b, rv.gv[0], grad_v)) != OC_OK) return rc;
+    OC_HIP(hipMemsetAsync(rv.gks, 0, sizeof(float) * std::max(b->Stot, 1), b->stream));
+    oc::StepScalars s = batch_scalars(b);
+    s.warm = 0;  // the adjoint assembly must not read or keep a warm start
+    int cur = 0;
+    for (int seg = nseg - 1; seg >= 0; seg--) {
+        const int n0 = seg * every, n1 = std::min(n0 + every, nsteps);
+        if ((rc = batch_restore(b, rv.ckpt + (size_t)seg * 3 * v3)) != OC_OK) return rc;
+        for (int n = n0; n < n1; n++) {
+            float* slot = rv.seg + (size_t)(n - n0) * 3 * v3;
+            const size_t bytes = sizeof(float) * v3;
+            OC_HIP(hipMemcpyAsync(slot, b->dc.x, bytes, hipMemcpyDeviceToDevice, b->stream));
+            OC_HIP(hipMemcpyAsync(slot + v3, b->dc.v, bytes, hipMemcpyDeviceToDevice, b->stream));
+            OC_HIP(hipGraphLaunch(b->graph, b->stream));
+            OC_HIP(hipMemcpyAsync(slot + 2 * v3, b->dc.dv, bytes, hipMemcpyDeviceToDevice, b->stream));
+        }
+        for (int n = n1 - 1; n >= n0; n--) {
+            float* slot = rv.seg + (size_t)(n - n0) * 3 * v3;
+            oc::DevBatch ca = b->dc;
+            ca.x = slot;
+            ca.v = slot + v3;
+            ca.dv = rv.mu;
+            oc::BatchAdjBufs ab;
+            ab.gx_out = rv.gx[cur];
+            ab.gv_out = rv.gv[cur];
+            ab.gvt = rv.gvt;
+            ab.dvsol = slot + 2 * v3;
+            ab.gx_in = rv.gx[1 - cur];
+            ab.gv_in = rv.gv[1 - cur];
+            ab.gks = rv.gks;
+            OC_HIP(oc::launch_batch_reset(b->dc, b->stream));
+            OC_HIP(oc::launch_batch_adjoint(ca, s, ab, b->sched.block_threads, b->grid, b->stream));
+            cur = 1 - cur;
+        }
+    }
+    if ((rc = batch_restore(b, rv.ckpt + (size_t)nseg * 3 * v3)) != OC_OK) return rc;
+    if (grad_ks) {
+        OC_HIP(hipMemcpyAsync(grad_ks, rv.gks, sizeof(float) * b->Stot, hipMemcpyDeviceToHost, b->stream));
+        OC_HIP(hipStreamSynchronize(b->stream));
+    }
+    if (grad_x0 && (rc = batch_download_aos(b, rv.gx[cur], grad_x0)) != OC_OK) return rc;
+    if (grad_v0 && (rc = batch_download_aos(b, rv.gv[cur], grad_v0)) != OC_OK) return rc;
+    OC_HIP(hipStreamSynchronize(b->stream));
     return OC_OK;
 }
 

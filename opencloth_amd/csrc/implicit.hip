@@ -390,7 +390,95 @@ __global__ __launch_bounds__(256) void k_update(DevCloth c, StepScalars s) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// adjoint of one step
+// ---------------------------------------------------------------------------
+// Adjoint solve start: gvt = gv' + h gx', r = d = S gvt, mu = 0, partial r.r
+template <int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_adj_init(DevCloth c, StepScalars s, AdjointBufs a) {
+    constexpr int NT = TX * TY;
+    __shared__ float ws[NT / 64];
+    const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+    const int tile = tile_of_block(blockIdx.x, c.nt);
+    const int bx = tile % c.ntx, by = tile / c.ntx;
+    const int gi = bx * TX + lx, gj = by * TY + ly;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c.ctl[0] = 0;
+        c.ctl[1] = 0;
+        c.ctl[2] = 0;
+    }
+    float t = 0.0f;
+    if (gi < c.nx && gj < c.ny) {
+        const int N = c.N, i = gj * c.nx + gi;
+        V3 gx = ld3(a.gx_out, N, i), gv = ld3(a.gv_out, N, i);
+        V3 g = {gv.x + s.h * gx.x, gv.y + s.h * gx.y, gv.z + s.h * gx.z};
+        st3(a.gvt, N, i, g);
+        V3 r = c.pin[i] ? V3{0.0f, 0.0f, 0.0f} : g;
+        st3(c.r, N, i, r);
+        st3(c.d0, N, i, r);
+        st3(c.dv, N, i, V3{0.0f, 0.0f, 0.0f});
+        t = dot3(r, r);
+    }
+    float p = tile_sum<NT>(t, ws);
+    if (threadIdx.x == 0) c.del_part[tile] = p;
+}
+
+// gx_in = gx' + dJ/dx, gv_in = gvt + dJ/dv, gks += dJ/dks (added by each
+// spring's p1).  c.x, c.v: pre-step state; c.dv: the step's solution.
+template <int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_adj_gather(DevCloth c, StepScalars s, AdjointBufs a) {
+    const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+    const int tile = tile_of_block(blockIdx.x, c.nt);
+    const int bx = tile % c.ntx, by = tile / c.ntx;
+    const int gi = bx * TX + lx, gj = by * TY + ly;
+    if (gi >= c.nx || gj >= c.ny) return;
+    const int N = c.N, i = gj * c.nx + gi;
+    const V3 xi = ld3(c.x, N, i), vi = ld3(c.v, N, i), mi = ld3(a.mu, N, i), pi = ld3(c.dv, N, i);
+    const V3 wi = add3(vi, pi);
+    V3 gx = ld3(a.gx_out, N, i), gv = ld3(a.gvt, N, i);
+    for (int k = 0; k < c.wa; k++) {
+        const int sp = c.ainc_s[k * N + i];
+        if (sp < 0) break;
+        const uint16_t m = c.ainc_m[k * N + i];
+        const int j = i + slot_dj(m) * c.nx + slot_di(m);
+        const V3 xj = ld3(c.x, N, j), vj = ld3(c.v, N, j), mj = ld3(a.mu, N, j), pj = ld3(c.dv, N, j);
+        const V3 wj = add3(vj, pj);
+        const bool second = (m >> 8) & 1u;
+        float ge[3], gvv[3], gk;
+        if (second)
+            spring_adjoint(xj, xi, vj, vi, mj, mi, wj, wi, pj, pi, c.L0[sp], c.ks[sp], c.kd[sp], s.h,
+                           s.hh, ge, gvv, gk);
+        else
+            spring_adjoint(xi, xj, vi, vj, mi, mj, wi, wj, pi, pj, c.L0[sp], c.ks[sp], c.kd[sp], s.h,
+                           s.hh, ge, gvv, gk);
+        if (second) {
+            gx = {gx.x - ge[0], gx.y - ge[1], gx.z - ge[2]};
+            gv = {gv.x - gvv[0], gv.y - gvv[1], gv.z - gvv[2]};
+        } else {
+            gx = {gx.x + ge[0], gx.y + ge[1], gx.z + ge[2]};
+            gv = {gv.x + gvv[0], gv.y + gvv[1], gv.z + gvv[2]};
+            a.gks[sp] = a.gks[sp] + gk;
+        }
+    }
+    st3(a.gx_in, N, i, gx);
+    st3(a.gv_in, N, i, gv);
+}
+
 }  // namespace
+
+hipError_t launch_adj_init(const DevCloth& c, const StepScalars& s, const AdjointBufs& a,
+                           hipStream_t st) {
+    hipLaunchKernelGGL((k_adj_init<kTileX, kTileY>), dim3(c.nt), dim3(kTileX * kTileY), 0, st, c, s,
+                       a);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_gather(const DevCloth& c, const StepScalars& s, const AdjointBufs& a,
+                             hipStream_t st) {
+    hipLaunchKernelGGL((k_adj_gather<kTileX, kTileY>), dim3(c.nt), dim3(kTileX * kTileY), 0, st, c,
+                       s, a);
+    return hipGetLastError();
+}
 
 hipError_t launch_forces(const DevCloth& c, const StepScalars& s, hipStream_t st) {
     hipLaunchKernelGGL((k_forces<kTileX, kTileY>), dim3(c.nt), dim3(kTileX * kTileY), 0, st, c, s);

@@ -124,5 +124,44 @@ __device__ inline float partial_sum(const float* part, int nt, float* bc) {
     return *bc;
 }
 
+// Per-spring adjoint terms (oracle.c spring_adjoint, same expression order):
+// derivative of h mu.f + h^2 mu.K v' + h mu.D dv with respect to e = x1-x2
+// (ge), to v1 (gvv) and to ks (gk).
+__device__ inline void spring_adjoint(V3 xa, V3 xb, V3 va, V3 vb, V3 ua, V3 ub, V3 wa, V3 wb,
+                                      V3 pa, V3 pb, float L0, float ks, float Kd, float h, float hh,
+                                      float* ge, float* gvv, float& gk) {
+    const float e[3] = {xa.x - xb.x, xa.y - xb.y, xa.z - xb.z};
+    const float dvel[3] = {va.x - vb.x, va.y - vb.y, va.z - vb.z};
+    const float u[3] = {ua.x - ub.x, ua.y - ub.y, ua.z - ub.z};
+    const float w[3] = {wa.x - wb.x, wa.y - wb.y, wa.z - wb.z};
+    const float wp[3] = {pa.x - pb.x, pa.y - pb.y, pa.z - pb.z};
+    const float l = sqrtf(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    const float il = 1.0f / l;
+    const float il2 = il * il, il3 = il2 * il, il4 = il2 * il2, il5 = il4 * il;
+    const float ue = u[0] * e[0] + u[1] * e[1] + u[2] * e[2];
+    const float we = w[0] * e[0] + w[1] * e[1] + w[2] * e[2];
+    const float wpe = wp[0] * e[0] + wp[1] * e[1] + wp[2] * e[2];
+    const float uw = u[0] * w[0] + u[1] * w[1] + u[2] * w[2];
+    const float dve = dvel[0] * e[0] + dvel[1] * e[1] + dvel[2] * e[2];
+    const float phi = -ks * (l - L0) + Kd * (dve * il);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float dphi = -ks * e[c] * il + Kd * (dvel[c] * il - dve * e[c] * il3);
+        const float A = h * ((ue * il) * dphi + phi * (u[c] * il - ue * e[c] * il3));
+        const float B = hh * ks * L0 *
+                        (-uw * e[c] * il3 - (u[c] * we + w[c] * ue) * il3 + 3.0f * ue * we * e[c] * il5);
+        const float Cc = h * Kd * ((u[c] * wpe + wp[c] * ue) * il2 - 2.0f * ue * wpe * e[c] * il4);
+        ge[c] = A + B + Cc;
+    }
+    V3 f;
+    float K[6], P[6];
+    spring_terms(xa, xb, va, vb, L0, ks, Kd, f, K, P);
+    const V3 ku = symmv(K, V3{u[0], u[1], u[2]});
+    gvv[0] = h * Kd * ue * e[0] * il2 + hh * ku.x;
+    gvv[1] = h * Kd * ue * e[1] * il2 + hh * ku.y;
+    gvv[2] = h * Kd * ue * e[2] * il2 + hh * ku.z;
+    gk = h * (ue * il) * (-(l - L0)) + hh * ((L0 * il - 1.0f) * uw - L0 * ue * we * il3);
+}
+
 }  // namespace dev
 }  // namespace oc

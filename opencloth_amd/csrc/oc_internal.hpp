@@ -90,6 +90,21 @@ hipError_t launch_cg_spmv(const DevCloth& c, const StepScalars& s, int k, hipStr
 hipError_t launch_cg_axpy(const DevCloth& c, const StepScalars& s, int k, hipStream_t st);
 hipError_t launch_cg_finish(const DevCloth& c, const StepScalars& s, hipStream_t st);
 hipError_t launch_update(const DevCloth& c, const StepScalars& s, hipStream_t st);
+// Buffers of one adjoint step (SoA vectors of the cloth's size).
+struct AdjointBufs {
+    const float* gx_out;  // dL/dx after the step
+    const float* gv_out;  // dL/dv after the step
+    float* gvt;           // gv' + h gx'
+    const float* mu;      // adjoint solve
+    float* gx_in;         // dL/dx before the step
+    float* gv_in;         // dL/dv before the step
+    float* gks;           // dL/dks, accumulated (per spring)
+};
+hipError_t launch_adj_init(const DevCloth& c, const StepScalars& s, const AdjointBufs& a,
+                           hipStream_t st);
+hipError_t launch_adj_gather(const DevCloth& c, const StepScalars& s, const AdjointBufs& a,
+                             hipStream_t st);
+
 // Tile shape of the implicit kernels.
 constexpr int kTileX = 32;
 constexpr int kTileY = 8;
@@ -128,6 +143,19 @@ struct DevBatch {
     int* inst_status;
 };
 
+// Buffers of one batched adjoint step (SoA, stride Ntot).
+struct BatchAdjBufs {
+    const float* gx_out;
+    const float* gv_out;
+    float* gvt;
+    const float* dvsol;  // the forward step's solution
+    float* gx_in;
+    float* gv_in;
+    float* gks;
+};
+
+hipError_t launch_batch_adjoint(const DevBatch& c, const StepScalars& s, const BatchAdjBufs& a,
+                                int block_threads, int grid, hipStream_t st);
 hipError_t launch_batch_reset(const DevBatch& c, hipStream_t st);
 hipError_t launch_batch_step(const DevBatch& c, const StepScalars& s, int block_threads,
                              int grid, hipStream_t st);

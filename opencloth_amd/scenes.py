@@ -216,6 +216,7 @@ class BatchScene:
     pinned: np.ndarray  # (sum N,) uint8
     params: StepParams
     steps: int
+    extra: dict = field(default_factory=dict)
 
     @property
     def n_cloths(self) -> int:
@@ -273,3 +274,52 @@ def batch_scene(n_cloths: int = 4096, sides=(16, 128), seed: int = 0, steps: int
     p = StepParams(dt=dt, cg_max_iters=cg_max_iters, cg_tol=cg_tol, cg_tol_mode=TOL_RELATIVE)
     return BatchScene(f"batch_{n_cloths}", side.copy(), side.copy(), X, np.zeros_like(X),
                       springs, nspr.astype(np.int32), pinned, p, steps)
+
+
+def smooth_target(X: np.ndarray, seed: int, amplitude: float = 0.2, n_waves: int = 4) -> np.ndarray:
+    """X displaced by a sum of n_waves random sinusoids: wave k has a unit
+    displacement direction d_k, wave vector q_k in the cloth plane (|q_k| in
+    [0.5, 2]) and phase p_k; each has the given amplitude."""
+    u = uniform(seed, 6 * n_waves).reshape(n_waves, 6)
+    out = X.astype(np.float64).copy()
+    for k in range(n_waves):
+        th, ph = 2 * math.pi * u[k, 0], math.acos(2 * u[k, 1] - 1)
+        d = np.array([math.sin(ph) * math.cos(th), math.cos(ph), math.sin(ph) * math.sin(th)])
+        ang, mag = 2 * math.pi * u[k, 2], 0.5 + 1.5 * u[k, 3]
+        q = mag * np.array([math.cos(ang), 0.0, math.sin(ang)])
+        phase = 2 * math.pi * u[k, 4]
+        out += amplitude * np.sin(X.astype(np.float64) @ q + phase)[:, None] * d[None, :]
+    return out.astype(np.float32)
+
+
+def diff_scene(n: int = 128, steps: int = 120, seed: int = 0, ks: float = 50.0) -> Scene:
+    """Differentiable rollout: n x n cloth (top edge pinned), dt 1/60, CG fixed
+    at 40 iterations, per-spring ks (all `ks` to start); target = rest grid
+    displaced by smooth_target(seed).  Target in extra['target']."""
+    X = grid_positions(n, n, 4.0, 0.0)
+    springs = grid_springs(n, n, X, ks=(ks, ks, ks))
+    pinned = np.zeros(n * n, np.uint8)
+    pinned[:n] = 1
+    p = StepParams(dt=1.0 / 60.0, cg_max_iters=40, cg_tol=0.0, cg_tol_mode=TOL_NONE)
+    return Scene(f"diff_{n}x{n}", n, n, X, np.zeros_like(X), springs, pinned, p, steps,
+                 extra={"target": smooth_target(X, seed)})
+
+
+def batch_diff_scene(n_cloths: int = 512, sides=(32, 96), steps: int = 60, seed: int = 0,
+                     ks_range=(20.0, 200.0), cg_tol: float = 1e-6) -> BatchScene:
+    """Batched differentiable rollouts: n_cloths cloths (top edge pinned),
+    sides uniform in [sides], per-spring ks log-uniform in ks_range, dt 1/60,
+    CG at most 40 iterations with relative tolerance cg_tol; per-cloth smooth
+    targets (smooth_target, seed + 1 + cloth).  Targets in `extra`."""
+    b = batch_scene(n_cloths=n_cloths, sides=sides, seed=seed, steps=steps, cg_tol=cg_tol)
+    u = uniform(seed + 99991, len(b.springs))
+    b.springs["Ks"] = np.exp(np.log(ks_range[0]) + u * (np.log(ks_range[1]) - np.log(ks_range[0]))).astype(np.float32)
+    b.pinned[:] = 0
+    targets = np.empty_like(b.X)
+    for c in range(n_cloths):
+        v0, n = int(b.voff[c]), int(b.nx[c])
+        b.pinned[v0:v0 + n] = 1
+        targets[v0:v0 + n * n] = smooth_target(b.X[v0:v0 + n * n], seed + 1 + c)
+    b.name = f"batch_diff_{n_cloths}"
+    b.extra = {"target": targets}
+    return b

@@ -79,6 +79,8 @@ class Oracle:
         L.orc_cg_reference_blockdiag.argtypes = [C.c_int, _P, _P, _P, C.c_int, C.c_double]
         L.orc_step.argtypes = [_P, _P, _P, _P, _P, _P]
         L.orc_rollout.argtypes = [_P, _P, _P, _P, _P, C.c_int, _P, _P]
+        L.orc_set_ks.argtypes = [_P, _P]
+        L.orc_step_adjoint.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
 
     # -- arrays --------------------------------------------------------------
     def arr(self, a) -> np.ndarray:
@@ -215,6 +217,49 @@ class System:
         st = o.L.orc_rollout(self.h, _ptr(X), _ptr(V), _ptr(DV), C.byref(prm), nsteps,
                              _ptr(hist) if hist is not None else None, _ptr(iters))
         return X, V, DV, hist, iters, st
+
+
+    def set_ks(self, ks):
+        """Per-spring stiffness in the oracle's precision."""
+        k = np.ascontiguousarray(ks, np.float64)
+        self.o.L.orc_set_ks(self.h, _ptr(k))
+
+    def step_adjoint(self, X, V, DV, prm: Params, gx_out, gv_out, gks):
+        """Adjoint of one step at pre-step state (X, V) with solution DV.
+        Returns (gx_in, gv_in, mu, iters); gks (S,) is accumulated in place."""
+        o = self.o
+        X = o.arr(X); V = o.arr(V); DV = o.arr(DV)
+        gxo = o.arr(gx_out); gvo = o.arr(gv_out)
+        gx = np.zeros((self.N, 3), o.dtype); gv = np.zeros((self.N, 3), o.dtype)
+        mu = np.zeros((self.N, 3), o.dtype)
+        st = C.c_int(0)
+        it = o.L.orc_step_adjoint(self.h, _ptr(X), _ptr(V), _ptr(DV), C.byref(prm), _ptr(gxo),
+                                  _ptr(gvo), _ptr(gx), _ptr(gv), _ptr(gks), _ptr(mu), C.byref(st))
+        return gx, gv, mu, it
+
+    def rollout_forward(self, X0, V0, prm: Params, nsteps):
+        """Rollout keeping every pre-step state and solution: (X_T, V_T, states)."""
+        o = self.o
+        X = o.arr(X0).copy(); V = o.arr(V0).copy(); DV = np.zeros((self.N, 3), o.dtype)
+        states = []
+        for _ in range(nsteps):
+            Xn, Vn = X.copy(), V.copy()
+            self.step(X, V, DV, prm)
+            states.append((Xn, Vn, DV.copy()))
+        return X, V, states
+
+    def rollout_grad(self, X0, V0, prm: Params, nsteps, gx_T, gv_T=None, states=None):
+        """Gradients of a scalar L through nsteps given dL/dx_T (and dL/dv_T):
+        returns (dL/dks, dL/dx0, dL/dv0)."""
+        o = self.o
+        if states is None:
+            _, _, states = self.rollout_forward(X0, V0, prm, nsteps)
+        gks = np.zeros(self.S, o.dtype)
+        gx = o.arr(gx_T).copy()
+        gv = np.zeros((self.N, 3), o.dtype) if gv_T is None else o.arr(gv_T).copy()
+        for Xn, Vn, DVn in reversed(states):
+            gx, gv, _, _ = self.step_adjoint(Xn, Vn, DVn, prm, gx, gv, gks)
+        return gks, gx, gv
 
 
 def cg_blockdiag(o: Oracle, A, b, x0, i_max, eps, literal=False):

@@ -181,6 +181,7 @@ EXPORT int orc_grid_springs(int nx, int ny, const float* X, float ks_struct, flo
 typedef struct {
     int N, S, P;
     orc_spring* springs;
+    real* ks; /* per-spring stiffness in the oracle's precision (orc_set_ks) */
     uint8_t* pinned;
     int* pair_a; /* lower vertex of each pair */
     int* pair_b;
@@ -209,6 +210,8 @@ EXPORT orc_sys* orc_create(int N, int S, const orc_spring* springs, const uint8_
     s->S = S;
     s->springs = (orc_spring*)malloc(sizeof(orc_spring) * (S > 0 ? S : 1));
     memcpy(s->springs, springs, sizeof(orc_spring) * S);
+    s->ks = (real*)malloc(sizeof(real) * (S > 0 ? S : 1));
+    for (int i = 0; i < S; i++) s->ks[i] = (real)springs[i].Ks;
     s->pinned = (uint8_t*)malloc(N > 0 ? N : 1);
     if (pinned)
         memcpy(s->pinned, pinned, N);
@@ -313,6 +316,7 @@ EXPORT orc_sys* orc_create(int N, int S, const orc_spring* springs, const uint8_
 EXPORT void orc_destroy(orc_sys* s) {
     if (!s) return;
     free(s->springs);
+    free(s->ks);
     free(s->pinned);
     free(s->pair_a);
     free(s->pair_b);
@@ -325,6 +329,12 @@ EXPORT void orc_destroy(orc_sys* s) {
 }
 
 EXPORT int orc_num_pairs(const orc_sys* s) { return s->P; }
+
+/* Per-spring stiffness in the oracle's own precision (FP64 finite
+ * differences in ks need steps finer than float resolution). */
+EXPORT void orc_set_ks(orc_sys* s, const double* ks) {
+    for (int i = 0; i < s->S; i++) s->ks[i] = (real)ks[i];
+}
 EXPORT void orc_pairs(const orc_sys* s, int* a, int* b, int* spring_pair) {
     memcpy(a, s->pair_a, sizeof(int) * s->P);
     memcpy(b, s->pair_b, sizeof(int) * s->P);
@@ -384,7 +394,7 @@ EXPORT void orc_spring_terms(const orc_sys* s, const real* X, const real* V, rea
         const orc_spring* sp = &s->springs[i];
         real f[3], K[6], P[6];
         spring_terms(X + 3 * sp->p1, X + 3 * sp->p2, V + 3 * sp->p1, V + 3 * sp->p2,
-                     (real)sp->rest_length, (real)sp->Ks, (real)sp->Kd, f, K, P);
+                     (real)sp->rest_length, s->ks[sp - s->springs], (real)sp->Kd, f, K, P);
         if (Fs) memcpy(Fs + 3 * i, f, sizeof(f));
         if (Ks6) memcpy(Ks6 + 6 * i, K, sizeof(K));
         if (P6) memcpy(P6 + 6 * i, P, sizeof(P));
@@ -414,7 +424,7 @@ EXPORT void orc_forces_scatter(const orc_sys* s, const real* X, const real* V, d
         const orc_spring* sp = &s->springs[i];
         real f[3];
         spring_terms(X + 3 * sp->p1, X + 3 * sp->p2, V + 3 * sp->p1, V + 3 * sp->p2,
-                     (real)sp->rest_length, (real)sp->Ks, (real)sp->Kd, f, NULL, NULL);
+                     (real)sp->rest_length, s->ks[sp - s->springs], (real)sp->Kd, f, NULL, NULL);
         if (!s->pinned[sp->p1])
             for (int c = 0; c < 3; c++) F[3 * sp->p1 + c] += f[c];
         if (!s->pinned[sp->p2])
@@ -437,7 +447,7 @@ EXPORT void orc_forces_gather(const orc_sys* s, const real* X, const real* V, do
             const orc_spring* sp = &s->springs[s->inc_s[e]];
             real f[3], K[6];
             spring_terms(X + 3 * sp->p1, X + 3 * sp->p2, V + 3 * sp->p1, V + 3 * sp->p2,
-                         (real)sp->rest_length, (real)sp->Ks, (real)sp->Kd, f, K, NULL);
+                         (real)sp->rest_length, s->ks[sp - s->springs], (real)sp->Kd, f, K, NULL);
             real w[3] = {V[3 * sp->p1 + 0] - V[3 * sp->p2 + 0], V[3 * sp->p1 + 1] - V[3 * sp->p2 + 1],
                          V[3 * sp->p1 + 2] - V[3 * sp->p2 + 2]};
             real kw[3];
@@ -470,11 +480,11 @@ EXPORT void orc_forces_gather(const orc_sys* s, const real* X, const real* V, do
  * applies, IE:674); BW:768-770's -Kd*(dp1+dp2+dp3) has the opposite sign
  * with BW's own negative Kd and is not followed.  df/dx is the reference's
  * stiffness Jacobian K (IE:671). */
-static void spring_block(const orc_spring* sp, const real* X, const real* V, real h, real hh,
+static void spring_block(const orc_sys* s, const orc_spring* sp, const real* X, const real* V, real h, real hh,
                          real* f, real* K, real* C) {
     real P[6];
     spring_terms(X + 3 * sp->p1, X + 3 * sp->p2, V + 3 * sp->p1, V + 3 * sp->p2,
-                 (real)sp->rest_length, (real)sp->Ks, (real)sp->Kd, f, K, P);
+                 (real)sp->rest_length, s->ks[sp - s->springs], (real)sp->Kd, f, K, P);
     real kd = (real)sp->Kd;
     for (int c = 0; c < 6; c++) C[c] = h * (kd * P[c]) + hh * K[c];
 }
@@ -494,7 +504,7 @@ EXPORT void orc_assemble(const orc_sys* s, const real* X, const real* V, const o
         for (int c = 0; c < 6; c++) Cp[6 * p + c] = 0;
     for (int i = 0; i < s->S; i++) {
         real f[3], K[6], C[6];
-        spring_block(&s->springs[i], X, V, h, hh, f, K, C);
+        spring_block(s, &s->springs[i], X, V, h, hh, f, K, C);
         int p = s->spring_pair[i];
         for (int c = 0; c < 6; c++) Cp[6 * p + c] += C[c];
     }
@@ -505,7 +515,7 @@ EXPORT void orc_assemble(const orc_sys* s, const real* X, const real* V, const o
         for (int e = s->inc_ptr[i]; e < s->inc_ptr[i + 1]; e++) {
             const orc_spring* sp = &s->springs[s->inc_s[e]];
             real f[3], K[6], C[6];
-            spring_block(sp, X, V, h, hh, f, K, C);
+            spring_block(s, sp, X, V, h, hh, f, K, C);
             real w[3] = {V[3 * sp->p1 + 0] - V[3 * sp->p2 + 0], V[3 * sp->p1 + 1] - V[3 * sp->p2 + 1],
                          V[3 * sp->p1 + 2] - V[3 * sp->p2 + 2]};
             real kw[3];
@@ -833,4 +843,116 @@ EXPORT int orc_rollout(const orc_sys* s, real* X, real* V, real* DV, const orc_p
         if (Xhist) memcpy(Xhist + (size_t)k * 3 * s->N, X, sizeof(real) * 3 * s->N);
     }
     return all;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint of one implicit step                                              */
+/* ------------------------------------------------------------------------ */
+/* Forward step (orc_step): A(x) dv = b(x,v) on the free vertices, with
+ *   A = M - h D(x) - h^2 K(x),  b = h f(x,v) + h^2 K(x) v,
+ *   v' = v + dv,  x' = x + h v'.
+ * Given gx' = dL/dx', gv' = dL/dv':
+ *   gvt = gv' + h gx'
+ *   A mu = S gvt           (filtered CG on the same symmetric matrix)
+ *   J(x,v,ks) = h mu.f + h^2 mu.K v' + h mu.D dv   (mu, v', dv held fixed)
+ *   gx = gx' + dJ/dx,  gv = gvt + dJ/dv,  gks += dJ/dks.
+ * Per spring (e = x1-x2, u = mu1-mu2, w = v'1-v'2, w' = dv1-dv2, l = |e|):
+ *   mu.f     -> phi (u.e)/l,  phi = -ks (l-L0) + Kd (dvel.e)/l
+ *   mu.K v'  -> ks [(L0/l - 1)(u.w) - L0 (u.e)(w.e)/l^3]
+ *   mu.D dv  -> Kd (u.e)(w'.e)/l^2
+ * plus the explicit v in K v (dJ/dv1 += h^2 K u).  Each vertex gathers its
+ * springs in the forward's order; a spring's ks gradient is added by its p1.
+ * The implicit-function form assumes the forward solve converged.
+ * Pinned by: tests/test_oracle_adjoint.py -- central finite differences of
+ * FP64 rollouts in ks, v0 and x0 directions and the dot-product test. */
+static void spring_adjoint(const real* xa, const real* xb, const real* va, const real* vb,
+                           const real* ua, const real* ub, const real* wa, const real* wb,
+                           const real* pa, const real* pb, real L0, real ks, real Kd, real h,
+                           real hh, real* ge, real* gvv, real* gk) {
+    real e[3], dvel[3], u[3], w[3], wp[3];
+    for (int c = 0; c < 3; c++) {
+        e[c] = xa[c] - xb[c];
+        dvel[c] = va[c] - vb[c];
+        u[c] = ua[c] - ub[c];
+        w[c] = wa[c] - wb[c];
+        wp[c] = pa[c] - pb[c];
+    }
+    real l = len3(e);
+    real il = (real)1 / l;
+    real il2 = il * il, il3 = il2 * il, il4 = il2 * il2, il5 = il4 * il;
+    real ue = dot3(u, e), we = dot3(w, e), wpe = dot3(wp, e), uw = dot3(u, w), dve = dot3(dvel, e);
+    real phi = -ks * (l - L0) + Kd * (dve * il);
+    for (int c = 0; c < 3; c++) {
+        real dphi = -ks * e[c] * il + Kd * (dvel[c] * il - dve * e[c] * il3);
+        real A = h * ((ue * il) * dphi + phi * (u[c] * il - ue * e[c] * il3));
+        real B = hh * ks * L0 * (-uw * e[c] * il3 - (u[c] * we + w[c] * ue) * il3 +
+                                 (real)3 * ue * we * e[c] * il5);
+        real Cc = h * Kd * ((u[c] * wpe + wp[c] * ue) * il2 - (real)2 * ue * wpe * e[c] * il4);
+        ge[c] = A + B + Cc;
+    }
+    real f[3], K[6], P[6];
+    spring_terms(xa, xb, va, vb, L0, ks, Kd, f, K, P);
+    real ku[3];
+    symmv(K, u, ku);
+    for (int c = 0; c < 3; c++) gvv[c] = h * Kd * ue * e[c] * il2 + hh * ku[c];
+    *gk = h * (ue * il) * (-(l - L0)) + hh * ((L0 * il - (real)1) * uw - L0 * ue * we * il3);
+}
+
+/* One step's adjoint at the pre-step state (X, V) whose solve gave DV.
+ * gks (S values) is accumulated; gx_in/gv_in are overwritten; mu_out (may be
+ * NULL) receives the adjoint solve.  Returns the adjoint CG iterations. */
+EXPORT int orc_step_adjoint(const orc_sys* s, const real* X, const real* V, const real* DV,
+                            const orc_params* prm, const real* gx_out, const real* gv_out,
+                            real* gx_in, real* gv_in, real* gks, real* mu_out, int* status) {
+    int N = s->N;
+    real h = (real)prm->h, hh = h * h;
+    real* gvt = (real*)malloc(sizeof(real) * 3 * N);
+    real* mu = (real*)calloc(3 * (size_t)N, sizeof(real));
+    real* B = (real*)malloc(sizeof(real) * 3 * N);
+    real* Cp = (real*)malloc(sizeof(real) * 6 * (s->P > 0 ? s->P : 1));
+    real* Vp = (real*)malloc(sizeof(real) * 3 * N);
+    for (int i = 0; i < 3 * N; i++) {
+        gvt[i] = gv_out[i] + h * gx_out[i];
+        Vp[i] = V[i] + DV[i];
+    }
+    orc_assemble(s, X, V, prm, B, Cp, NULL);
+    pair_ctx pc = {s, Cp, prm};
+    int it = cg_core(N, s->pinned, pair_mv, pair_dot, &pc, prm->max_it, (real)prm->tol,
+                     prm->tol_mode, gvt, mu, status, NULL);
+    for (int i = 0; i < N; i++) {
+        real gx[3] = {gx_out[3 * i], gx_out[3 * i + 1], gx_out[3 * i + 2]};
+        real gv[3] = {gvt[3 * i], gvt[3 * i + 1], gvt[3 * i + 2]};
+        for (int e = s->inc_ptr[i]; e < s->inc_ptr[i + 1]; e++) {
+            int sid = s->inc_s[e];
+            const orc_spring* sp = &s->springs[sid];
+            int a = sp->p1, b = sp->p2;
+            real ge[3], gvv[3], gk;
+            spring_adjoint(X + 3 * a, X + 3 * b, V + 3 * a, V + 3 * b, mu + 3 * a, mu + 3 * b,
+                           Vp + 3 * a, Vp + 3 * b, DV + 3 * a, DV + 3 * b, (real)sp->rest_length,
+                           s->ks[sid], (real)sp->Kd, h, hh, ge, gvv, &gk);
+            if (a == i) {
+                for (int c = 0; c < 3; c++) {
+                    gx[c] += ge[c];
+                    gv[c] += gvv[c];
+                }
+                gks[sid] += gk;
+            } else {
+                for (int c = 0; c < 3; c++) {
+                    gx[c] -= ge[c];
+                    gv[c] -= gvv[c];
+                }
+            }
+        }
+        for (int c = 0; c < 3; c++) {
+            gx_in[3 * i + c] = gx[c];
+            gv_in[3 * i + c] = gv[c];
+        }
+    }
+    if (mu_out) memcpy(mu_out, mu, sizeof(real) * 3 * N);
+    free(gvt);
+    free(mu);
+    free(B);
+    free(Cp);
+    free(Vp);
+    return it;
 }

@@ -1,0 +1,141 @@
+"""GPU adjoint of the implicit step (oc_cloth_rollout_forward/backward).
+
+Bars: gradients bit-exact against the FP32 oracle's adjoint with the
+kernels' reduction order (small cloth); on the differentiable-rollout
+workload (128x128, 120 steps, 40 CG iterations, checkpoint every 16) the FP32
+GPU gradient against the FP64 oracle's adjoint (relative error of the
+gradient vectors) and against FP64 central finite differences stored in
+tests/golden/adjoint_fd_128.json; a torch.autograd.Function wrapper gives the
+same gradients as the C ABI."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from opencloth_amd import scenes
+from tests.helpers import oracle_params, say
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def random_ks(sc, seed):
+    rng = np.random.default_rng(seed)
+    return (20.0 * np.exp(np.log(10.0) * rng.random(len(sc.springs)))).astype(np.float32)
+
+
+def gpu_grad(sc, ks, V0, steps, every):
+    from opencloth_amd.cloth import ImplicitCloth
+
+    cl = ImplicitCloth.from_scene(sc)
+    cl.set_spring_stiffness(ks)
+    cl.set_state(sc.X, V0)
+    cl.rollout_forward(steps, every)
+    XT = cl.X
+    gx = (2.0 * (XT.astype(np.float32) - sc.extra["target"])).astype(np.float32)
+    gks, gx0, gv0 = cl.rollout_backward(gx)
+    return XT, gx, gks, gx0, gv0
+
+
+def test_adjoint_bitwise_against_oracle32():
+    from oracle import Oracle
+
+    sc = scenes.diff_scene(n=24, steps=10, seed=1)
+    ks = random_ks(sc, 2)
+    V0 = (0.2 * np.random.default_rng(3).standard_normal(sc.X.shape)).astype(np.float32)
+    V0[sc.pinned.astype(bool)] = 0
+    say("adjoint 24x24 x10 bitwise")
+    XT, gx, gks, gx0, gv0 = gpu_grad(sc, ks, V0, 10, 4)
+    o = Oracle(32)
+    s = o.system(sc.n_vertices, sc.springs, sc.pinned)
+    s.set_ks(ks.astype(np.float64))
+    prm = oracle_params(sc.params, sc.nx, sc.ny)
+    XTr, _, states = s.rollout_forward(sc.X, V0, prm, 10)
+    assert np.array_equal(XT.view(np.uint32), XTr.view(np.uint32))
+    rks, rx0, rv0 = s.rollout_grad(sc.X, V0, prm, 10, gx, states=states)
+    assert np.array_equal(gks.view(np.uint32), rks.view(np.uint32)), np.abs(gks - rks).max()
+    assert np.array_equal(gv0.view(np.uint32), rv0.view(np.uint32))
+    assert np.array_equal(gx0.view(np.uint32), rx0.view(np.uint32))
+
+
+@pytest.fixture(scope="module")
+def c4():
+    """The differentiable-rollout workload with the GPU gradient."""
+    with open(os.path.join(HERE, "golden", "adjoint_fd_128.json")) as f:
+        fx = json.load(f)
+    sc = scenes.diff_scene(n=fx["n"], steps=fx["steps"], seed=fx["seed"])
+    ks = random_ks(sc, fx["ks_seed"])
+    V0 = np.zeros_like(sc.X)
+    say("adjoint 128x128 x120 on the GPU")
+    XT, gx, gks, gx0, gv0 = gpu_grad(sc, ks, V0, fx["steps"], fx["checkpoint_every"])
+    return dict(fx=fx, sc=sc, ks=ks, V0=V0, XT=XT, gks=gks, gv0=gv0, gx0=gx0)
+
+
+def test_gradient_against_fp64_adjoint(c4):
+    from oracle import Oracle
+
+    fx, sc = c4["fx"], c4["sc"]
+    o = Oracle(64)
+    s = o.system(sc.n_vertices, sc.springs, sc.pinned)
+    s.set_ks(c4["ks"].astype(np.float64))
+    prm = oracle_params(sc.params, sc.nx, sc.ny, gpu_order=False)
+    XT, _, states = s.rollout_forward(sc.X, c4["V0"], prm, fx["steps"])
+    gk64, _, gv64 = s.rollout_grad(sc.X, c4["V0"], prm, fx["steps"], 2 * (XT - sc.extra["target"]),
+                                   states=states)
+    ek = np.linalg.norm(c4["gks"] - gk64) / np.linalg.norm(gk64)
+    ev = np.linalg.norm(c4["gv0"] - gv64) / np.linalg.norm(gv64)
+    say(f"relative gradient error vs FP64 adjoint: ks {ek:.3e}, v0 {ev:.3e}")
+    assert ek < 1e-3 and ev < 1e-3
+
+
+@pytest.mark.parametrize("eps,bar", [("1e-05", 1e-3), ("0.001", 2e-2)])
+def test_gradient_against_finite_differences(c4, eps, bar):
+    """32 random (ks, v0) directions; eps 1e-5 is the tight reference, the
+    workload's eps 1e-3 carries O(eps^2) truncation up to ~1.2e-2."""
+    fx = c4["fx"]
+    dirs = np.random.default_rng(fx["dir_seed"])
+    worst = 0.0
+    for k, fd in enumerate(fx["fd"][eps]):
+        dk = dirs.standard_normal(len(c4["ks"])) * c4["ks"].astype(np.float64)
+        dv = dirs.standard_normal(c4["V0"].shape)
+        an = float(c4["gks"].astype(np.float64) @ dk + (c4["gv0"].astype(np.float64) * dv).sum())
+        worst = max(worst, abs(an - fd) / abs(fd))
+    say(f"eps {eps}: {len(fx['fd'][eps])} directions, worst relative error vs FP64 FD {worst:.3e}")
+    assert worst < bar
+
+
+def test_torch_autograd_function():
+    torch = pytest.importorskip("torch")
+    from opencloth_amd.cloth import ImplicitCloth
+
+    sc = scenes.diff_scene(n=20, steps=8, seed=4)
+    cl = ImplicitCloth.from_scene(sc)
+    target = torch.from_numpy(sc.extra["target"])
+
+    class Rollout(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, ks, v0):
+            cl.set_spring_stiffness(ks.detach().numpy())
+            cl.set_state(sc.X, v0.detach().numpy())
+            cl.rollout_forward(8, 3)
+            return torch.from_numpy(cl.X)
+
+        @staticmethod
+        def backward(ctx, gx):
+            gks, _, gv0 = cl.rollout_backward(gx.contiguous().numpy())
+            return torch.from_numpy(gks), torch.from_numpy(gv0)
+
+    ks = torch.from_numpy(random_ks(sc, 5)).requires_grad_(True)
+    v0 = torch.zeros(sc.X.shape, dtype=torch.float32, requires_grad=True)
+    xT = Rollout.apply(ks, v0)
+    loss = ((xT - target) ** 2).sum()
+    loss.backward()
+    # the same gradients straight through the C ABI
+    cl.set_spring_stiffness(ks.detach().numpy())
+    cl.set_state(sc.X, np.zeros_like(sc.X))
+    cl.rollout_forward(8, 3)
+    gks, _, gv0 = cl.rollout_backward(2 * (cl.X - sc.extra["target"]))
+    assert torch.allclose(ks.grad, torch.from_numpy(gks), rtol=1e-6, atol=0)
+    assert torch.allclose(v0.grad, torch.from_numpy(gv0), rtol=1e-6, atol=1e-12)
+    assert float(loss) > 0
