@@ -145,6 +145,85 @@ def bench_batch(steps: int, warmup: int, n_cloths: int = 4096, plain_steps: int 
     }
 
 
+def bench_batch_adjoint(n_cloths: int = 512, steps: int = 60, every: int = 10,
+                        sample=(0, 255, 511)) -> dict:
+    """Batched forward + adjoint: n_cloths cloths, sides uniform in [32,96],
+    per-spring ks log-uniform [20,200], top edges pinned, 60-step rollouts to
+    per-cloth smooth targets, CG <= 40 iterations (relative tol 1e-6) in
+    both directions, checkpoints every 10 steps.  The plain form runs the same
+    rollouts one cloth at a time through oc_cloth_rollout_forward/backward.
+    Gradient error: FP32 batch against the FP64 oracle adjoint on sampled
+    cloths (relative 2-norm of dL/dks and dL/dv0)."""
+    from opencloth_amd import scenes
+    from opencloth_amd.batch import ClothBatch
+    from opencloth_amd.cloth import ImplicitCloth
+    from opencloth_amd.hiptime import Event
+
+    sc = scenes.batch_diff_scene(n_cloths=n_cloths, steps=steps)
+    target = sc.extra["target"]
+    b = ClothBatch.from_scene(sc)
+    sched = b.autotune()
+    b.rollout_forward(steps, every)  # warm-up
+    XT, _ = b.get_state()
+    gx = (2.0 * (XT - target)).astype(np.float32)
+    b.rollout_backward(gx, n_springs=len(sc.springs))
+    b.set_state(sc.X, sc.V)
+    e0, e1 = Event(), Event()
+    e0.record()
+    b.rollout_forward(steps, every)
+    XT, _ = b.get_state()
+    gx = (2.0 * (XT - target)).astype(np.float32)
+    gks, _, gv0 = b.rollout_backward(gx, n_springs=len(sc.springs))
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1)
+    b.close()
+
+    plain = [ImplicitCloth.from_scene(sc.cloth(c)) for c in range(sc.n_cloths)]
+    for c, cl in enumerate(plain[:2]):
+        cl.rollout_forward(steps, every)
+        cl.rollout_backward(np.zeros((cl.N, 3), np.float32))
+    p0, p1 = Event(), Event()
+    p0.record()
+    for c, cl in enumerate(plain):
+        v0, n = int(sc.voff[c]), cl.N
+        cl.set_state(sc.X[v0:v0 + n], sc.V[v0:v0 + n])
+        cl.rollout_forward(steps, every)
+        xt = cl.X
+        cl.rollout_backward(2.0 * (xt - target[v0:v0 + n]))
+    p1.record()
+    p1.synchronize()
+    ms_plain = p0.elapsed_ms(p1)
+    for cl in plain:
+        cl.close()
+
+    from oracle import Oracle, make_params
+
+    o = Oracle(64)
+    errs = []
+    for c in sample:
+        cs = sc.cloth(c)
+        v0, s0, n = int(sc.voff[c]), int(sc.soff[c]), cs.n_vertices
+        s = o.system(n, cs.springs, cs.pinned)
+        prm = make_params(cs.params)
+        XTr, _, states = s.rollout_forward(cs.X, cs.V, prm, steps)
+        rks, _, rv0 = s.rollout_grad(cs.X, cs.V, prm, steps, 2 * (XTr - target[v0:v0 + n]), states=states)
+        errs.append({"cloth": int(c), "side": int(cs.nx),
+                     "ks_rel_err": float(np.linalg.norm(gks[s0:s0 + len(cs.springs)] - rks) / np.linalg.norm(rks)),
+                     "v0_rel_err": float(np.linalg.norm(gv0[v0:v0 + n] - rv0) / np.linalg.norm(rv0))})
+    N = int(sc.X.shape[0])
+    return {
+        "workload": f"{sc.n_cloths} cloths, sides U[32,96], {N} vertices, {len(sc.springs)} springs, "
+                    f"{steps}-step rollouts, checkpoint every {every}, CG <= 40 it rel tol 1e-6",
+        "ms_fwd_adj": ms,
+        "vertex_steps_per_s_fwd_adj": N * steps / (ms * 1e-3),
+        "plain_ms_fwd_adj": ms_plain,
+        "speedup_vs_plain": ms_plain / ms,
+        "schedule": {"block_threads": sched[0], "vertices_per_thread": sched[1]},
+        "grad_error_vs_fp64": errs,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,6 +301,7 @@ def main():
     }
     if not args.no_batch:
         out["batch"] = bench_batch(args.batch_steps, 2)
+        out["batch_adjoint"] = bench_batch_adjoint()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(out), flush=True)

@@ -93,3 +93,16 @@ def test_single_step_adjoint_dot_product(o64):
     gx, gv, _, _ = s.step_adjoint(X, V, dv, prm, wx, wv, gks)
     rhs = float((gx * ux).sum() + (gv * uv).sum())
     assert abs(lhs - rhs) <= 1e-7 * abs(lhs)
+
+
+def test_fd_fixture_consistent():
+    """The stored FP64 finite differences of the 128x128 rollout workload agree
+    with the stored FP64 adjoint projections (eps 1e-5: truncation below 1e-5)."""
+    import json
+    import os
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "adjoint_fd_128.json")) as f:
+        fx = json.load(f)
+    fd = np.asarray(fx["fd"]["1e-05"]); adj = np.asarray(fx["adjoint_fp64"])
+    assert len(fd) == 32
+    assert np.all(np.abs(fd - adj) <= 1e-5 * np.abs(fd))
