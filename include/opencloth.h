@@ -143,6 +143,59 @@ int oc_cloth_synchronize(oc_cloth cloth);
 /* Status of the last solve: CG iterations and OC_SOLVE_* bits; synchronises. */
 int oc_cloth_last_status(oc_cloth cloth, int32_t* iters, int32_t* status);
 
+/* ---- post-step passes of StepPhysics (IE:774-779) ----------------------- */
+/* Ellipsoid collider: transform and inverse as column-major 4x4 matrices
+ * (glm::mat4, IE:541-544), object-space center and radius (IE:85-86). */
+typedef struct {
+    float transform[16];
+    float inverse[16];
+    float center[3];
+    float radius;
+} oc_ellipsoid;
+
+/* Push vertices inside the ellipsoid to its surface and zero their velocity.
+ * Replaces EllipsoidCollision (IE:727-751). */
+int oc_cloth_ellipsoid_collision(oc_cloth cloth, const oc_ellipsoid* ellipsoid);
+/* Move half the excess length of every spring longer than slack*rest into
+ * the velocities.  Replaces ApplyProvotDynamicInverse (IE:702-726, slack 1;
+ * BW:800-823, slack 1.01). */
+int oc_cloth_provot(oc_cloth cloth, float slack);
+/* nsteps full steps: implicit step, then the ellipsoid pass (when given) and
+ * the Provot pass (when slack > 0).  Replaces StepPhysics (IE:774-779). */
+int oc_cloth_step_physics(oc_cloth cloth, int32_t nsteps, const oc_ellipsoid* ellipsoid,
+                          float provot_slack);
+
+/* ---- position Verlet (VL) ---------------------------------------------- */
+typedef struct oc_verlet_s* oc_verlet;
+
+/* Scalars of the Verlet demo: mass (VL:102), gravity as an acceleration
+ * (VL:101, the force is gravity*mass), damping (DEFAULT_DAMPING, VL:97),
+ * dt (VL:104) and OC_FLAG_GROUND_CLAMP (VL:440-442). */
+typedef struct {
+    float mass;
+    float gravity[3];
+    float damping;
+    float dt;
+    int32_t flags;
+} oc_verlet_params;
+
+/* x_last0 may be NULL (start at rest: x_last = x).  Springs as for
+ * oc_cloth_create.  Replaces the Verlet demo's InitGL state (VL:240-328). */
+int oc_verlet_create(int32_t nx, int32_t ny, const float* x0, const float* x_last0,
+                     const oc_spring* springs, int32_t n_springs, const uint8_t* pinned,
+                     const oc_verlet_params* params, oc_verlet* out);
+int oc_verlet_destroy(oc_verlet verlet);
+int oc_verlet_set_stream(oc_verlet verlet, void* hip_stream);
+int oc_verlet_set_state(oc_verlet verlet, const float* x, const float* x_last);
+int oc_verlet_get_state(oc_verlet verlet, float* x, float* x_last);
+/* Forces with Verlet velocities (x - x_last)/dt.  Replaces ComputeForces(dt)
+ * (VL:448-484); f_out host AoS; synchronises. */
+int oc_verlet_compute_forces(oc_verlet verlet, float* f_out);
+/* nsteps of ComputeForces + IntegrateVerlet (VL:428-444) fused per vertex,
+ * then the ellipsoid pass (x_last = x on contact, VL:509-533) when given.
+ * Replaces StepPhysics of the Verlet demo (VL:557-562). */
+int oc_verlet_step(oc_verlet verlet, int32_t nsteps, const oc_ellipsoid* ellipsoid);
+
 /* ---- differentiable rollouts (no counterpart in the reference) ---------- */
 /* Per-spring stiffness Ks (n_springs floats, the spring list's order); the
  * reference fixes Ks per spring type (IE:310-312). */

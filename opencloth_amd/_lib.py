@@ -53,6 +53,16 @@ SIGNATURES = {
     "oc_cloth_num_pairs": (C.c_int, [_P, _P]),
     "oc_cloth_get_pairs": (C.c_int, [_P, _P, _P]),
     "oc_cloth_get_system": (C.c_int, [_P, _P, _P]),
+    "oc_cloth_ellipsoid_collision": (C.c_int, [_P, _P]),
+    "oc_cloth_provot": (C.c_int, [_P, _F]),
+    "oc_cloth_step_physics": (C.c_int, [_P, _I, _P, _F]),
+    "oc_verlet_create": (C.c_int, [_I, _I, _P, _P, _P, _I, _P, _P, _P]),
+    "oc_verlet_destroy": (C.c_int, [_P]),
+    "oc_verlet_set_stream": (C.c_int, [_P, _P]),
+    "oc_verlet_set_state": (C.c_int, [_P, _P, _P]),
+    "oc_verlet_get_state": (C.c_int, [_P, _P, _P]),
+    "oc_verlet_compute_forces": (C.c_int, [_P, _P]),
+    "oc_verlet_step": (C.c_int, [_P, _I, _P]),
     "oc_batch_create": (C.c_int, [_I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "oc_batch_destroy": (C.c_int, [_P]),
     "oc_batch_set_stream": (C.c_int, [_P, _P]),
@@ -82,6 +92,31 @@ class OcParams(C.Structure):
         ("cg_max_iters", C.c_int32), ("cg_tol", C.c_float), ("cg_tol_mode", C.c_int32),
         ("warm_start", C.c_int32), ("flags", C.c_int32),
     ]
+
+
+class OcEllipsoid(C.Structure):
+    """oc_ellipsoid (include/opencloth.h)."""
+
+    _fields_ = [("transform", C.c_float * 16), ("inverse", C.c_float * 16),
+                ("center", C.c_float * 3), ("radius", C.c_float)]
+
+    @classmethod
+    def from_dict(cls, e: dict) -> "OcEllipsoid":
+        o = cls()
+        for k in range(16):
+            o.transform[k] = float(e["M"][k])
+            o.inverse[k] = float(e["Minv"][k])
+        for k in range(3):
+            o.center[k] = float(e["center"][k])
+        o.radius = float(e["radius"])
+        return o
+
+
+class OcVerletParams(C.Structure):
+    """oc_verlet_params (include/opencloth.h)."""
+
+    _fields_ = [("mass", C.c_float), ("gravity", C.c_float * 3), ("damping", C.c_float),
+                ("dt", C.c_float), ("flags", C.c_int32)]
 
 
 class OcBatchSchedule(C.Structure):

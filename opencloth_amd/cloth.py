@@ -13,7 +13,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import OcParams, check, lib, ptr
+from ._lib import OcEllipsoid, OcParams, OcVerletParams, check, lib, ptr
 from .scenes import SPRING_DTYPE, Scene, StepParams
 
 
@@ -157,6 +157,22 @@ class ImplicitCloth:
         check(lib().oc_cloth_last_status(self.h, C.byref(it), C.byref(st)))
         return it.value, st.value
 
+    # -- post-step passes (StepPhysics, IE:774-779) ---------------------------
+    def ellipsoid_collision(self, ell: dict):
+        """EllipsoidCollision (IE:727-751)."""
+        e = OcEllipsoid.from_dict(ell)
+        check(lib().oc_cloth_ellipsoid_collision(self.h, C.byref(e)))
+
+    def provot(self, slack: float = 1.0):
+        """ApplyProvotDynamicInverse (IE:702-726; slack 1.01 for BW:808)."""
+        check(lib().oc_cloth_provot(self.h, float(slack)))
+
+    def step_physics(self, nsteps: int = 1, ell: dict | None = None, provot_slack: float = 0.0):
+        """StepPhysics (IE:774-779): implicit step, ellipsoid, Provot."""
+        e = None if ell is None else OcEllipsoid.from_dict(ell)
+        check(lib().oc_cloth_step_physics(self.h, int(nsteps), C.byref(e) if e is not None else None,
+                                          float(provot_slack)))
+
     # -- differentiable rollouts ---------------------------------------------
     def set_spring_stiffness(self, ks):
         k = np.ascontiguousarray(ks, np.float32)
@@ -194,3 +210,60 @@ class ImplicitCloth:
         Cb = np.zeros((self.P, 6), np.float32); b = np.zeros((self.N, 3), np.float32)
         check(lib().oc_cloth_get_system(self.h, ptr(Cb), ptr(b)))
         return Cb, b
+
+
+class VerletCloth:
+    """Position-Verlet cloth (oc_verlet), the Verlet demo's step
+    (OpenCloth_Verlet/OpenCloth_Verlet/main.cpp:428-562)."""
+
+    def __init__(self, nx, ny, X, springs, pinned=None, mass=1.0, gravity=(0.0, -9.81, 0.0),
+                 damping=-0.0125, dt=1.0 / 60.0, ground_clamp=True, X_last=None):
+        self.nx, self.ny = int(nx), int(ny)
+        self.N = self.nx * self.ny
+        p = OcVerletParams()
+        p.mass = mass
+        p.gravity[0], p.gravity[1], p.gravity[2] = gravity
+        p.damping = damping
+        p.dt = dt
+        p.flags = 1 if ground_clamp else 0
+        self._p = p
+        X = _vec(X, self.N)
+        XL = None if X_last is None else _vec(X_last, self.N)
+        springs = np.ascontiguousarray(springs, dtype=SPRING_DTYPE)
+        pins = None if pinned is None else np.ascontiguousarray(pinned, np.uint8)
+        h = C.c_void_p()
+        check(lib().oc_verlet_create(self.nx, self.ny, ptr(X), ptr(XL), ptr(springs), len(springs),
+                                     ptr(pins), C.byref(p), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().oc_verlet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_state(self):
+        X = np.zeros((self.N, 3), np.float32); XL = np.zeros((self.N, 3), np.float32)
+        check(lib().oc_verlet_get_state(self.h, ptr(X), ptr(XL)))
+        return X, XL
+
+    def set_state(self, X=None, X_last=None):
+        Xa = None if X is None else _vec(X, self.N)
+        XLa = None if X_last is None else _vec(X_last, self.N)
+        check(lib().oc_verlet_set_state(self.h, ptr(Xa), ptr(XLa)))
+
+    def compute_forces(self):
+        """ComputeForces(dt) (VL:448-484)."""
+        F = np.zeros((self.N, 3), np.float32)
+        check(lib().oc_verlet_compute_forces(self.h, ptr(F)))
+        return F
+
+    def step(self, nsteps: int = 1, ell: dict | None = None):
+        """StepPhysics of the Verlet demo (VL:557-562)."""
+        e = None if ell is None else OcEllipsoid.from_dict(ell)
+        check(lib().oc_verlet_step(self.h, int(nsteps), C.byref(e) if e is not None else None))

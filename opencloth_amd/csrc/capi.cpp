@@ -433,6 +433,34 @@ int oc_cloth_profile_steps(oc_cloth c, int32_t nsteps, float* ms, int32_t* launc
     return rc;
 }
 
+int oc_cloth_ellipsoid_collision(oc_cloth c, const oc_ellipsoid* e) {
+    if (!c || !e) return fail(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    OC_HIP(oc::launch_ellipsoid(c->dc.x, c->dc.v, c->dc.N, *e, 0, c->stream));
+    return OC_OK;
+}
+
+int oc_cloth_provot(oc_cloth c, float slack) {
+    if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
+    if (!(slack > 0.0f)) return fail(OC_ERR_INVALID_ARGUMENT, "slack must be positive");
+    OC_HIP(oc::launch_provot(c->dc, slack, c->stream));
+    return OC_OK;
+}
+
+int oc_cloth_step_physics(oc_cloth c, int32_t nsteps, const oc_ellipsoid* e, float slack) {
+    if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
+    if (nsteps < 0) return fail(OC_ERR_INVALID_ARGUMENT, "nsteps < 0");
+    const oc::StepScalars s = scalars(c, c->prm.dt);
+    for (int k = 0; k < nsteps; k++) {
+        OC_HIP(oc::launch_assemble(c->dc, s, c->stream));
+        OC_HIP(oc::launch_cg(c->dc, s, c->stream));
+        OC_HIP(oc::launch_update(c->dc, s, c->stream));
+        if (e) OC_HIP(oc::launch_ellipsoid(c->dc.x, c->dc.v, c->dc.N, *e, 0, c->stream));
+        if (slack > 0.0f) OC_HIP(oc::launch_provot(c->dc, slack, c->stream));
+    }
+    c->assembled = false;
+    return OC_OK;
+}
+
 int oc_cloth_synchronize(oc_cloth c) {
     if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
     OC_HIP(hipStreamSynchronize(c->stream));

@@ -105,6 +105,32 @@ hipError_t launch_adj_init(const DevCloth& c, const StepScalars& s, const Adjoin
 hipError_t launch_adj_gather(const DevCloth& c, const StepScalars& s, const AdjointBufs& a,
                              hipStream_t st);
 
+// Post-step passes on an implicit cloth (post.hip).
+hipError_t launch_ellipsoid(float* x, float* w, int N, const oc_ellipsoid& e, int mode,
+                            hipStream_t st);
+hipError_t launch_provot(const DevCloth& c, float slack, hipStream_t st);
+
+// Position Verlet (post.hip): positions rotate through three buffers.
+struct DevVerlet {
+    int nx, ny, N, wa;
+    const float* x;   // current
+    const float* xl;  // previous
+    float* xn;        // next
+    float* f;
+    const uint8_t* pin;
+    const int32_t* ainc_s;
+    const uint16_t* ainc_m;
+    const float* L0;
+    const float* ks;
+    const float* kd;
+};
+struct VerletScalars {
+    float mass, gx, gy, gz, damping, dt;
+    int flags;
+};
+hipError_t launch_verlet_forces(const DevVerlet& c, const VerletScalars& s, hipStream_t st);
+hipError_t launch_verlet_step(const DevVerlet& c, const VerletScalars& s, hipStream_t st);
+
 // Tile shape of the implicit kernels.
 constexpr int kTileX = 32;
 constexpr int kTileY = 8;

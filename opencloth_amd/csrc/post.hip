@@ -1,0 +1,167 @@
+// Post-step passes of StepPhysics and the position-Verlet step (gfx950).
+//
+//   k_ellipsoid : EllipsoidCollision (IE:727-751, VL:509-533), per vertex
+//   k_provot    : ApplyProvotDynamicInverse (IE:702-726), per-vertex gather
+//                 of the springs' velocity corrections in list order (the
+//                 reference only reads X there, so the gather equals its
+//                 sequential scatter bit for bit)
+//   k_verlet    : ComputeForces(dt) + IntegrateVerlet (VL:428-484) fused per
+//                 vertex; reads x, x_last and writes the next positions into
+//                 a third buffer, so no thread sees a half-updated neighbour
+//
+// Expressions follow the reference's GLM order, built with -ffp-contract=off.
+#include "oc_device.hpp"
+
+namespace oc {
+
+namespace {
+
+using namespace dev;
+
+__global__ __launch_bounds__(256) void k_ellipsoid(float* x, float* w, int N, oc_ellipsoid e,
+                                                   int mode) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    const V3 p = ld3(x, N, i);
+    float d0[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const float t = e.inverse[0 * 4 + r] * p.x + e.inverse[1 * 4 + r] * p.y +
+                        e.inverse[2 * 4 + r] * p.z + e.inverse[3 * 4 + r] * 1.0f;
+        d0[r] = t - e.center[r];
+    }
+    const float dist = sqrtf(d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2]);
+    if (!(dist < 1.0f)) return;
+#pragma unroll
+    for (int c = 0; c < 3; c++) d0[c] = ((e.radius - dist) * d0[c]) / dist;
+    float delta[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        float t[3] = {e.transform[0 * 4 + a], e.transform[1 * 4 + a], e.transform[2 * 4 + a]};
+        const float tt = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+#pragma unroll
+        for (int c = 0; c < 3; c++) t[c] = t[c] / tt;
+        delta[a] = d0[0] * t[0] + d0[1] * t[1] + d0[2] * t[2];
+    }
+    const V3 q = {p.x + delta[0], p.y + delta[1], p.z + delta[2]};
+    st3(x, N, i, q);
+    st3(w, N, i, mode == 1 ? q : V3{0.0f, 0.0f, 0.0f});
+}
+
+// Velocity correction of one spring for its endpoints (IE:706-714).
+__device__ inline bool provot_delta(V3 x1, V3 x2, float rest, float slack, V3& dP) {
+    V3 dp = sub3(x1, x2);
+    const float sq = dp.x * dp.x + dp.y * dp.y + dp.z * dp.z;
+    float dist = sqrtf(sq);
+    const float thr = rest * slack;
+    if (!(dist > thr)) return false;
+    dist -= thr;
+    dist /= 2.0f;
+    const float inv = 1.0f / sqrtf(sq);
+    dP = {(dp.x * inv) * dist, (dp.y * inv) * dist, (dp.z * inv) * dist};
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_provot(DevCloth c, float slack) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= c.N) return;
+    const int N = c.N;
+    const V3 xi = ld3(c.x, N, i);
+    V3 v = ld3(c.v, N, i);
+    const bool pin_i = c.pin[i] != 0;
+    bool touched = false;
+    for (int k = 0; k < c.wa; k++) {
+        const int sp = c.ainc_s[k * N + i];
+        if (sp < 0) break;
+        const uint16_t m = c.ainc_m[k * N + i];
+        const int j = i + slot_dj(m) * c.nx + slot_di(m);
+        const bool second = (m >> 8) & 1u;
+        const V3 xj = ld3(c.x, N, j);
+        V3 dP;
+        if (!(second ? provot_delta(xj, xi, c.L0[sp], slack, dP)
+                     : provot_delta(xi, xj, c.L0[sp], slack, dP)))
+            continue;
+        const bool pin_j = c.pin[j] != 0;
+        if (!second) {  // i is p1: V[p1] -= dP unless p1 is pinned
+            if (!pin_i) {
+                v = sub3(v, dP);
+                touched = true;
+            }
+        } else {  // i is p2: V[p2] += dP when p1 is pinned or p2 is free
+            if (pin_j || !pin_i) {
+                v = add3(v, dP);
+                touched = true;
+            }
+        }
+    }
+    if (touched) st3(c.v, N, i, v);
+}
+
+// Force on vertex i with Verlet velocities (VL:448-484), gather in list order.
+__device__ inline V3 verlet_force(const DevVerlet& c, const VerletScalars& s, int i) {
+    const int N = c.N;
+    const V3 xi = ld3(c.x, N, i), xli = ld3(c.xl, N, i);
+    const V3 vi = {(xi.x - xli.x) / s.dt, (xi.y - xli.y) / s.dt, (xi.z - xli.z) / s.dt};
+    V3 F = {0.0f, 0.0f, 0.0f};
+    const bool pinned = c.pin[i] != 0;
+    if (!pinned) F = {F.x + s.gx * s.mass, F.y + s.gy * s.mass, F.z + s.gz * s.mass};
+    F = {F.x + s.damping * vi.x, F.y + s.damping * vi.y, F.z + s.damping * vi.z};
+    if (pinned) return F;
+    for (int k = 0; k < c.wa; k++) {
+        const int sp = c.ainc_s[k * N + i];
+        if (sp < 0) break;
+        const uint16_t m = c.ainc_m[k * N + i];
+        const int j = i + slot_dj(m) * c.nx + slot_di(m);
+        const V3 xj = ld3(c.x, N, j), xlj = ld3(c.xl, N, j);
+        const V3 vj = {(xj.x - xlj.x) / s.dt, (xj.y - xlj.y) / s.dt, (xj.z - xlj.z) / s.dt};
+        const bool second = (m >> 8) & 1u;
+        const V3 f = second ? spring_force(xj, xi, vj, vi, c.L0[sp], c.ks[sp], c.kd[sp])
+                            : spring_force(xi, xj, vi, vj, c.L0[sp], c.ks[sp], c.kd[sp]);
+        F = second ? sub3(F, f) : add3(F, f);
+    }
+    return F;
+}
+
+__global__ __launch_bounds__(256) void k_verlet_forces(DevVerlet c, VerletScalars s) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= c.N) return;
+    st3(c.f, c.N, i, verlet_force(c, s, i));
+}
+
+__global__ __launch_bounds__(256) void k_verlet(DevVerlet c, VerletScalars s) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= c.N) return;
+    const int N = c.N;
+    const V3 F = verlet_force(c, s, i);
+    const float c2 = (s.dt * s.dt) / s.mass;
+    const V3 x = ld3(c.x, N, i), xl = ld3(c.xl, N, i);
+    V3 xn = {x.x + (x.x - xl.x) + c2 * F.x, x.y + (x.y - xl.y) + c2 * F.y,
+             x.z + (x.z - xl.z) + c2 * F.z};
+    if ((s.flags & OC_FLAG_GROUND_CLAMP) && xn.y < 0.0f) xn.y = 0.0f;
+    st3(c.xn, N, i, xn);
+}
+
+}  // namespace
+
+hipError_t launch_ellipsoid(float* x, float* w, int N, const oc_ellipsoid& e, int mode,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(k_ellipsoid, dim3((N + 255) / 256), dim3(256), 0, st, x, w, N, e, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_provot(const DevCloth& c, float slack, hipStream_t st) {
+    hipLaunchKernelGGL(k_provot, dim3((c.N + 255) / 256), dim3(256), 0, st, c, slack);
+    return hipGetLastError();
+}
+
+hipError_t launch_verlet_forces(const DevVerlet& c, const VerletScalars& s, hipStream_t st) {
+    hipLaunchKernelGGL(k_verlet_forces, dim3((c.N + 255) / 256), dim3(256), 0, st, c, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_verlet_step(const DevVerlet& c, const VerletScalars& s, hipStream_t st) {
+    hipLaunchKernelGGL(k_verlet, dim3((c.N + 255) / 256), dim3(256), 0, st, c, s);
+    return hipGetLastError();
+}
+
+}  // namespace oc

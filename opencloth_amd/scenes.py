@@ -323,3 +323,47 @@ def batch_diff_scene(n_cloths: int = 512, sides=(32, 96), steps: int = 60, seed:
     b.name = f"batch_diff_{n_cloths}"
     b.extra = {"target": targets}
     return b
+
+
+def ellipsoid(translate=(0.0, 2.0, 0.0), angle_deg: float = 45.0, radius: float = 1.0) -> dict:
+    """The reference's collider (IE:541-544): translate * rotate(angle about x)
+    * scale(r, r, r/2), as column-major float32 4x4 matrices with the inverse;
+    object-space center 0 and radius 1 (IE:85-86).  Built here in float64 and
+    rounded, not with GLM's float matrix code."""
+    a = math.radians(angle_deg)
+    T = np.eye(4); T[:3, 3] = translate
+    R = np.eye(4); R[1, 1] = R[2, 2] = math.cos(a); R[1, 2] = -math.sin(a); R[2, 1] = math.sin(a)
+    S = np.diag([radius, radius, radius / 2, 1.0])
+    M = T @ R @ S
+    return {"M": M.T.astype(np.float32).ravel(), "Minv": np.linalg.inv(M).T.astype(np.float32).ravel(),
+            "center": np.zeros(3, np.float32), "radius": 1.0}
+
+
+def demo_scene(n: int = 21) -> Scene:
+    """The implicit demo's own scene (IE:59-62, IE:309-318, IE:434-546): 21x21
+    at height fullsize+1 = 5, Ks 0.75/0.75/0.95, Kd -0.25, mass 0.5, gravity
+    force -0.00981 (here acceleration -0.01962, so mass*g is the same float),
+    dt 1/60, CG 10 iterations with absolute tol 1e-3, ground clamp, pins
+    {0, numX}; the post-step passes use ellipsoid() and Provot slack 1."""
+    X = grid_positions(n, n, 4.0, 5.0)
+    springs = grid_springs(n, n, X, ks=(0.75, 0.75, 0.95), kd=(-0.25, -0.25, -0.25))
+    pinned = np.zeros(n * n, np.uint8)
+    pinned[0] = pinned[n - 1] = 1
+    g = float(np.float32(-0.00981) * np.float32(2.0))
+    p = StepParams(mass=0.5, gravity=(0.0, g, 0.0), dt=1.0 / 60.0, cg_max_iters=10, cg_tol=1e-3,
+                   cg_tol_mode=TOL_ABSOLUTE, ground_clamp=True)
+    return Scene(f"demo_{n}x{n}", n, n, X, np.zeros_like(X), springs, pinned, p, 600,
+                 extra={"ellipsoid": ellipsoid(), "provot_slack": 1.0})
+
+
+def verlet_demo_scene(n: int = 21) -> Scene:
+    """The Verlet demo's scene (VL:59-106, VL:240-328): height 5, Ks 50.75 /
+    50.75 / 50.95, Kd -0.25, mass 1, gravity -0.00981, damping -0.0125,
+    dt 1/60, ground clamp, pins {0, numX}, ellipsoid()."""
+    X = grid_positions(n, n, 4.0, 5.0)
+    springs = grid_springs(n, n, X, ks=(50.75, 50.75, 50.95), kd=(-0.25, -0.25, -0.25))
+    pinned = np.zeros(n * n, np.uint8)
+    pinned[0] = pinned[n - 1] = 1
+    p = StepParams(mass=1.0, gravity=(0.0, -0.00981, 0.0), dt=1.0 / 60.0, ground_clamp=True)
+    return Scene(f"verlet_demo_{n}x{n}", n, n, X, np.zeros_like(X), springs, pinned, p, 600,
+                 extra={"ellipsoid": ellipsoid(), "damping": -0.0125})

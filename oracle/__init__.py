@@ -80,6 +80,12 @@ class Oracle:
         L.orc_step.argtypes = [_P, _P, _P, _P, _P, _P]
         L.orc_rollout.argtypes = [_P, _P, _P, _P, _P, C.c_int, _P, _P]
         L.orc_set_ks.argtypes = [_P, _P]
+        L.orc_ellipsoid.argtypes = [_P, _P, _P, _P, _P, _P, C.c_float, C.c_int]
+        L.orc_provot.argtypes = [_P, _P, _P, C.c_double]
+        L.orc_verlet_forces.argtypes = [_P, _P, _P, C.c_double, C.c_double, _P, C.c_double, _P]
+        L.orc_verlet_integrate.argtypes = [_P, _P, _P, _P, C.c_double, C.c_double, C.c_int]
+        L.orc_verlet_step.argtypes = [_P, _P, _P, C.c_double, C.c_double, _P, C.c_double, C.c_int,
+                                      _P, _P, _P, C.c_float]
         L.orc_step_adjoint.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
 
     # -- arrays --------------------------------------------------------------
@@ -236,6 +242,53 @@ class System:
         it = o.L.orc_step_adjoint(self.h, _ptr(X), _ptr(V), _ptr(DV), C.byref(prm), _ptr(gxo),
                                   _ptr(gvo), _ptr(gx), _ptr(gv), _ptr(gks), _ptr(mu), C.byref(st))
         return gx, gv, mu, it
+
+    # -- post-step passes and Verlet -------------------------------------------
+    def ellipsoid(self, X, W, ell, mode):
+        """In place on X, W (this oracle's dtype).  ell: dict with M, Minv
+        (column-major 16), center (3), radius."""
+        M = np.ascontiguousarray(ell["M"], np.float32); Mi = np.ascontiguousarray(ell["Minv"], np.float32)
+        cen = np.ascontiguousarray(ell["center"], np.float32)
+        self.o.L.orc_ellipsoid(self.h, _ptr(X), _ptr(W), _ptr(M), _ptr(Mi), _ptr(cen),
+                               float(ell["radius"]), int(mode))
+
+    def provot(self, X, V, slack=1.0):
+        self.o.L.orc_provot(self.h, _ptr(X), _ptr(V), float(slack))
+
+    def verlet_forces(self, X, XL, dt, mass, g, damping):
+        o = self.o
+        X = o.arr(X); XL = o.arr(XL)
+        F = np.zeros((self.N, 3), o.dtype)
+        gg = np.asarray(g, np.float64)
+        o.L.orc_verlet_forces(self.h, _ptr(X), _ptr(XL), float(dt), float(mass), _ptr(gg),
+                              float(damping), _ptr(F))
+        return F
+
+    def verlet_rollout(self, X, XL, dt, mass, g, damping, flags, nsteps, ell=None):
+        o = self.o
+        X = o.arr(X).copy(); XL = o.arr(XL).copy()
+        gg = np.asarray(g, np.float64)
+        if ell is not None:
+            M = np.ascontiguousarray(ell["M"], np.float32); Mi = np.ascontiguousarray(ell["Minv"], np.float32)
+            cen = np.ascontiguousarray(ell["center"], np.float32); rad = float(ell["radius"])
+        for _ in range(nsteps):
+            if ell is None:
+                o.L.orc_verlet_step(self.h, _ptr(X), _ptr(XL), float(dt), float(mass), _ptr(gg),
+                                    float(damping), int(flags), None, None, None, 0.0)
+            else:
+                o.L.orc_verlet_step(self.h, _ptr(X), _ptr(XL), float(dt), float(mass), _ptr(gg),
+                                    float(damping), int(flags), _ptr(M), _ptr(Mi), _ptr(cen), rad)
+        return X, XL
+
+    def step_physics(self, X, V, DV, prm: Params, ell=None, provot_slack=0.0):
+        """StepPhysics (IE:774-779): implicit step, ellipsoid (zero velocity),
+        Provot (when slack > 0).  In place; returns (iters, status)."""
+        it, st = self.step(X, V, DV, prm)
+        if ell is not None:
+            self.ellipsoid(X, V, ell, 0)
+        if provot_slack > 0:
+            self.provot(X, V, provot_slack)
+        return it, st
 
     def rollout_forward(self, X0, V0, prm: Params, nsteps):
         """Rollout keeping every pre-step state and solution: (X_T, V_T, states)."""

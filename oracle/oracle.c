@@ -956,3 +956,138 @@ EXPORT int orc_step_adjoint(const orc_sys* s, const real* X, const real* V, cons
     free(Vp);
     return it;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Post-step passes of StepPhysics                                           */
+/* ------------------------------------------------------------------------ */
+
+/* EllipsoidCollision (IE:727-751; VL:509-533; PBD:846-870).  M, Minv:
+ * column-major 4x4 (glm::mat4, M[4*col + row]).  A vertex inside the unit
+ * sphere of the ellipsoid's object space is pushed to its surface scaled by
+ * radius; then mode 0 zeroes its velocity W (IE, PBD) and mode 1 sets its
+ * previous position W = X (VL:530).
+ * Pinned by: tests/test_oracle_post.py (outside points unchanged, points on
+ * the axis of a sphere pushed to the closed-form surface point, GLM order). */
+EXPORT void orc_ellipsoid(const orc_sys* s, real* X, real* W, const float* M, const float* Minv,
+                          const float* center, float radius, int mode) {
+    for (int i = 0; i < s->N; i++) {
+        real* x = X + 3 * i;
+        real d0[3];
+        for (int r = 0; r < 3; r++) {
+            real t = (real)Minv[0 * 4 + r] * x[0] + (real)Minv[1 * 4 + r] * x[1] +
+                     (real)Minv[2 * 4 + r] * x[2] + (real)Minv[3 * 4 + r] * (real)1;
+            d0[r] = t - (real)center[r];
+        }
+        real dist = len3(d0);
+        if (dist < (real)1) {
+            for (int c = 0; c < 3; c++) d0[c] = (((real)radius - dist) * d0[c]) / dist;
+            real delta[3];
+            for (int a = 0; a < 3; a++) {
+                real t[3] = {(real)M[0 * 4 + a], (real)M[1 * 4 + a], (real)M[2 * 4 + a]};
+                real tt = dot3(t, t);
+                for (int c = 0; c < 3; c++) t[c] = t[c] / tt;
+                delta[a] = dot3(d0, t);
+            }
+            for (int c = 0; c < 3; c++) x[c] += delta[c];
+            for (int c = 0; c < 3; c++) W[3 * i + c] = mode == 1 ? x[c] : (real)0;
+        }
+    }
+}
+
+/* ApplyProvotDynamicInverse (IE:702-726; BW:800-823 with slack 1.01): for
+ * every spring longer than slack*rest, in list order, half the excess along
+ * the spring is moved into the velocities V (never into X, so all springs
+ * see the same positions); a pinned p1 sends it all to p2, else a pinned p2
+ * skips p2's share.  Pins generalise the reference's {0, numX}.
+ * Pinned by: tests/test_oracle_post.py (excess-length closed form on a
+ * stretched pair, no change below the threshold, momentum of an unpinned
+ * system). */
+EXPORT void orc_provot(const orc_sys* s, const real* X, real* V, double slack) {
+    for (int i = 0; i < s->S; i++) {
+        const orc_spring* sp = &s->springs[i];
+        real dp[3] = {X[3 * sp->p1] - X[3 * sp->p2], X[3 * sp->p1 + 1] - X[3 * sp->p2 + 1],
+                      X[3 * sp->p1 + 2] - X[3 * sp->p2 + 2]};
+        real dist = len3(dp);
+        real thr = (real)sp->rest_length * (real)slack;
+        if (dist > thr) {
+            dist -= thr;
+            dist /= (real)2;
+            real inv = (real)1 / SQRT_(dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2]);
+            for (int c = 0; c < 3; c++) dp[c] = dp[c] * inv;
+            for (int c = 0; c < 3; c++) dp[c] *= dist;
+            if (s->pinned[sp->p1]) {
+                for (int c = 0; c < 3; c++) V[3 * sp->p2 + c] += dp[c];
+            } else if (s->pinned[sp->p2]) {
+                for (int c = 0; c < 3; c++) V[3 * sp->p1 + c] -= dp[c];
+            } else {
+                for (int c = 0; c < 3; c++) V[3 * sp->p1 + c] -= dp[c];
+                for (int c = 0; c < 3; c++) V[3 * sp->p2 + c] += dp[c];
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Position Verlet (VL)                                                      */
+/* ------------------------------------------------------------------------ */
+
+/* ComputeForces(dt) (VL:448-484): velocities (x - x_last)/dt
+ * (GetVerletVelocity, VL:445-447); gravity*mass on free vertices, damping*v on
+ * all (VL:455-458); spring forces scattered in list order, pinned endpoints
+ * skipped (VL:462-483).
+ * Pinned by: equality with orc_forces_scatter at v = (x - x_last)/dt plus
+ * the damping term (tests/test_oracle_post.py). */
+EXPORT void orc_verlet_forces(const orc_sys* s, const real* X, const real* XL, double dt,
+                              double mass, const double* g, double damping, real* F) {
+    real h = (real)dt, m = (real)mass, kd = (real)damping;
+    real gv[3] = {(real)g[0], (real)g[1], (real)g[2]};
+    real* V = (real*)malloc(sizeof(real) * 3 * (s->N > 0 ? s->N : 1));
+    for (int i = 0; i < 3 * s->N; i++) V[i] = (X[i] - XL[i]) / h;
+    for (int i = 0; i < s->N; i++) {
+        for (int c = 0; c < 3; c++) F[3 * i + c] = 0;
+        if (!s->pinned[i])
+            for (int c = 0; c < 3; c++) F[3 * i + c] += gv[c] * m;
+        for (int c = 0; c < 3; c++) F[3 * i + c] += kd * V[3 * i + c];
+    }
+    for (int i = 0; i < s->S; i++) {
+        const orc_spring* sp = &s->springs[i];
+        real f[3];
+        spring_terms(X + 3 * sp->p1, X + 3 * sp->p2, V + 3 * sp->p1, V + 3 * sp->p2,
+                     (real)sp->rest_length, s->ks[i], (real)sp->Kd, f, NULL, NULL);
+        if (!s->pinned[sp->p1])
+            for (int c = 0; c < 3; c++) F[3 * sp->p1 + c] += f[c];
+        if (!s->pinned[sp->p2])
+            for (int c = 0; c < 3; c++) F[3 * sp->p2 + c] -= f[c];
+    }
+    free(V);
+}
+
+/* IntegrateVerlet(dt) (VL:428-444): x' = x + (x - x_last) + (dt^2/mass) F,
+ * x_last' = x, then (flags bit0) y < 0 clamped to 0.
+ * Pinned by: free fall (no springs) against the closed form of the
+ * recurrence x_{n+1} = 2x_n - x_{n-1} + dt^2 g. */
+EXPORT void orc_verlet_integrate(const orc_sys* s, real* X, real* XL, const real* F, double dt,
+                                 double mass, int flags) {
+    real h = (real)dt;
+    real c2 = (h * h) / (real)mass;
+    for (int i = 0; i < s->N; i++) {
+        for (int c = 0; c < 3; c++) {
+            real buffer = X[3 * i + c];
+            X[3 * i + c] = X[3 * i + c] + (X[3 * i + c] - XL[3 * i + c]) + c2 * F[3 * i + c];
+            XL[3 * i + c] = buffer;
+        }
+        if ((flags & 1) && X[3 * i + 1] < 0) X[3 * i + 1] = 0;
+    }
+}
+
+/* StepPhysics of the Verlet demo (VL:557-562): forces, integrate, and the
+ * ellipsoid pass when M is given (mode 1). */
+EXPORT void orc_verlet_step(const orc_sys* s, real* X, real* XL, double dt, double mass,
+                            const double* g, double damping, int flags, const float* M,
+                            const float* Minv, const float* center, float radius) {
+    real* F = (real*)malloc(sizeof(real) * 3 * (s->N > 0 ? s->N : 1));
+    orc_verlet_forces(s, X, XL, dt, mass, g, damping, F);
+    orc_verlet_integrate(s, X, XL, F, dt, mass, flags);
+    if (M) orc_ellipsoid(s, X, XL, M, Minv, center, radius, 1);
+    free(F);
+}
