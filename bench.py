@@ -57,6 +57,20 @@ def read_traffic(path: str):
     return None
 
 
+def copy_probe(mib: int) -> int:
+    """One torch device-to-device copy of mib MiB (an elementwise copy kernel
+    that reads and writes mib MiB once each); returns the bytes copied."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    return n * 4
+
+
 def cpu_baseline(sc, budget_s: float = 12.0) -> dict:
     """FP32 CPU restatement (one core) timed on whole steps of the same
     workload until about budget_s seconds have passed."""
@@ -233,6 +247,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
     ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the timing, run one plain device copy of MIB MiB (a kernel of "
+                         "known HBM traffic, the scale for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -304,6 +321,8 @@ def main():
         out["batch_adjoint"] = bench_batch_adjoint()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc)
+    if args.copy_probe > 0:
+        out["copy_probe_bytes"] = copy_probe(args.copy_probe)
     print(json.dumps(out), flush=True)
 
 

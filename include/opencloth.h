@@ -55,7 +55,11 @@ typedef enum {
 #define OC_TOL_NONE 2     /* run exactly cg_max_iters iterations */
 
 /* oc_params.flags */
-#define OC_FLAG_GROUND_CLAMP 1 /* clamp y < 0 to 0 after the update (IE:695-697) */
+#define OC_FLAG_GROUND_CLAMP 1  /* clamp y < 0 to 0 after the update (IE:695-697) */
+#define OC_FLAG_CG_PERSISTENT 2 /* run all CG iterations of a solve in one cooperative
+                                   launch, resident blocks serving several tiles each
+                                   (same arithmetic and results, grid barriers between
+                                   phases); ignored where cooperative launch is absent */
 
 /* Spring record, identical to the reference's `struct Spring` (IE:68-73). */
 typedef struct {
@@ -195,6 +199,56 @@ int oc_verlet_compute_forces(oc_verlet verlet, float* f_out);
  * then the ellipsoid pass (x_last = x on contact, VL:509-533) when given.
  * Replaces StepPhysics of the Verlet demo (VL:557-562). */
 int oc_verlet_step(oc_verlet verlet, int32_t nsteps, const oc_ellipsoid* ellipsoid);
+
+/* ---- position-based dynamics (PBD) --------------------------------------- */
+/* PBD = OpenCloth_PositionBasedDynamics/OpenCloth_PositionBasedDynamics/main.cpp,
+ * built with USE_TRIANGLE_BENDING_CONSTRAINT (PBD:57). */
+typedef struct oc_pbd_s* oc_pbd;
+
+/* AddDistanceConstraint(p1, p2, k) (PBD:148-162). */
+typedef struct {
+    int32_t p1, p2;
+    float k;
+} oc_pbd_distance;
+/* AddBendingConstraint(p1, p2, p3, k), triangle form (PBD:164-178). */
+typedef struct {
+    int32_t p1, p2, p3;
+    float k;
+} oc_pbd_bending;
+
+typedef struct {
+    float mass;               /* per particle (PBD:114); inverse mass 1/mass, 0 if pinned (PBD:345-351) */
+    float gravity[3];         /* acceleration (PBD:112) */
+    float dt;                 /* PBD:74 */
+    float global_damping;     /* velocity factor per step (PBD:78), capped at 1 (PBD:395-396) */
+    float k_damp;             /* centre-of-mass damping (PBD:111), clamped to [0,1] (PBD:391-394) */
+    int32_t solver_iterations; /* PBD:107 */
+    int32_t flags;            /* OC_FLAG_GROUND_CLAMP: GroundCollision in the solver loop (PBD:887) */
+} oc_pbd_params;
+
+/* n_vertices particles at x0 (host AoS float3), pinned may be NULL.  Rest
+ * lengths come from x0, k is clamped to [0,1] and k' = 1-(1-k)^(1/ns) as in
+ * the reference's InitGL (PBD:148-178, PBD:383-396).  Each constraint family
+ * is split into colours (greedy, in list order) whose constraints touch
+ * disjoint particles; colours are projected one after another.  Replaces the
+ * PBD demo's InitGL state (PBD:310-464). */
+int oc_pbd_create(int32_t n_vertices, const float* x0, const uint8_t* pinned,
+                  const oc_pbd_distance* distance, int32_t n_distance,
+                  const oc_pbd_bending* bending, int32_t n_bending, const oc_pbd_params* params,
+                  oc_pbd* out);
+int oc_pbd_destroy(oc_pbd pbd);
+int oc_pbd_set_stream(oc_pbd pbd, void* hip_stream);
+int oc_pbd_set_state(oc_pbd pbd, const float* x, const float* v);
+int oc_pbd_get_state(oc_pbd pbd, float* x, float* v);
+/* Colours of the distance and bending constraints. */
+int oc_pbd_num_colors(oc_pbd pbd, int32_t* n_distance_colors, int32_t* n_bending_colors);
+/* nsteps of StepPhysics (PBD:915-925): gravity, global and centre-of-mass
+ * damping, prediction (PBD:601-668), solver_iterations passes of distance
+ * and bending projections and the ground clamp (PBD:876-889), the ellipsoid
+ * pass on the predicted positions when given (PBD:846-874), then
+ * V = (tmp_X - X)/dt, X = tmp_X (PBD:670-678). */
+int oc_pbd_step(oc_pbd pbd, int32_t nsteps, const oc_ellipsoid* ellipsoid);
+int oc_pbd_synchronize(oc_pbd pbd);
 
 /* ---- differentiable rollouts (no counterpart in the reference) ---------- */
 /* Per-spring stiffness Ks (n_springs floats, the spring list's order); the

@@ -19,6 +19,7 @@ OC_ERR_OUT_OF_MEMORY = -4
 OC_ERR_NO_DEVICE = -5
 
 SOLVE_NOT_CONVERGED, SOLVE_BREAKDOWN, SOLVE_NONFINITE = 1, 2, 4
+OC_FLAG_GROUND_CLAMP, OC_FLAG_CG_PERSISTENT = 1, 2
 
 # Every symbol include/opencloth.h declares, with (restype, argtypes).
 _P = C.c_void_p
@@ -63,6 +64,14 @@ SIGNATURES = {
     "oc_verlet_get_state": (C.c_int, [_P, _P, _P]),
     "oc_verlet_compute_forces": (C.c_int, [_P, _P]),
     "oc_verlet_step": (C.c_int, [_P, _I, _P]),
+    "oc_pbd_create": (C.c_int, [_I, _P, _P, _P, _I, _P, _I, _P, _P]),
+    "oc_pbd_destroy": (C.c_int, [_P]),
+    "oc_pbd_set_stream": (C.c_int, [_P, _P]),
+    "oc_pbd_set_state": (C.c_int, [_P, _P, _P]),
+    "oc_pbd_get_state": (C.c_int, [_P, _P, _P]),
+    "oc_pbd_num_colors": (C.c_int, [_P, _P, _P]),
+    "oc_pbd_step": (C.c_int, [_P, _I, _P]),
+    "oc_pbd_synchronize": (C.c_int, [_P]),
     "oc_batch_create": (C.c_int, [_I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "oc_batch_destroy": (C.c_int, [_P]),
     "oc_batch_set_stream": (C.c_int, [_P, _P]),
@@ -117,6 +126,14 @@ class OcVerletParams(C.Structure):
 
     _fields_ = [("mass", C.c_float), ("gravity", C.c_float * 3), ("damping", C.c_float),
                 ("dt", C.c_float), ("flags", C.c_int32)]
+
+
+class OcPbdParams(C.Structure):
+    """oc_pbd_params (include/opencloth.h)."""
+
+    _fields_ = [("mass", C.c_float), ("gravity", C.c_float * 3), ("dt", C.c_float),
+                ("global_damping", C.c_float), ("k_damp", C.c_float),
+                ("solver_iterations", C.c_int32), ("flags", C.c_int32)]
 
 
 class OcBatchSchedule(C.Structure):

@@ -13,8 +13,9 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import OcEllipsoid, OcParams, OcVerletParams, check, lib, ptr
-from .scenes import SPRING_DTYPE, Scene, StepParams
+from ._lib import OcEllipsoid, OcParams, OcPbdParams, OcVerletParams, check, lib, ptr
+from .scenes import (PBD_BENDING_DTYPE, PBD_DISTANCE_DTYPE, SPRING_DTYPE, PBDParams, Scene,
+                     StepParams)
 
 
 def to_oc_params(p: StepParams) -> OcParams:
@@ -26,7 +27,8 @@ def to_oc_params(p: StepParams) -> OcParams:
     q.cg_tol = p.cg_tol
     q.cg_tol_mode = p.cg_tol_mode
     q.warm_start = 1 if p.warm_start else 0
-    q.flags = 1 if p.ground_clamp else 0
+    q.flags = (_lib.OC_FLAG_GROUND_CLAMP if p.ground_clamp else 0) | (
+        _lib.OC_FLAG_CG_PERSISTENT if p.cg_persistent else 0)
     return q
 
 
@@ -267,3 +269,73 @@ class VerletCloth:
         """StepPhysics of the Verlet demo (VL:557-562)."""
         e = None if ell is None else OcEllipsoid.from_dict(ell)
         check(lib().oc_verlet_step(self.h, int(nsteps), C.byref(e) if e is not None else None))
+
+
+class PBDCloth:
+    """Position-based-dynamics cloth (oc_pbd), the PBD demo's step
+    (OpenCloth_PositionBasedDynamics/.../main.cpp:601-925) over given
+    distance and bending constraint lists."""
+
+    def __init__(self, X, distance, bending, pinned=None, params: PBDParams | None = None, V=None):
+        X = np.ascontiguousarray(X, np.float32)
+        self.N = len(X)
+        X = _vec(X, self.N)
+        self.params = params or PBDParams()
+        p = OcPbdParams()
+        p.mass = self.params.mass
+        p.gravity[0], p.gravity[1], p.gravity[2] = self.params.gravity
+        p.dt = self.params.dt
+        p.global_damping = self.params.global_damping
+        p.k_damp = self.params.k_damp
+        p.solver_iterations = self.params.solver_iterations
+        p.flags = _lib.OC_FLAG_GROUND_CLAMP if self.params.ground else 0
+        self._p = p
+        dist = np.ascontiguousarray(distance, dtype=PBD_DISTANCE_DTYPE)
+        bend = np.ascontiguousarray(bending, dtype=PBD_BENDING_DTYPE)
+        pins = None if pinned is None else np.ascontiguousarray(pinned, np.uint8)
+        if pins is not None and pins.shape != (self.N,):
+            raise ValueError("pinned must have one entry per particle")
+        h = C.c_void_p()
+        check(lib().oc_pbd_create(self.N, ptr(X), ptr(pins), ptr(dist), len(dist), ptr(bend),
+                                  len(bend), C.byref(p), C.byref(h)))
+        self.h = h
+        if V is not None:
+            self.set_state(V=V)
+
+    @classmethod
+    def from_scene(cls, sc) -> "PBDCloth":
+        return cls(sc.X, sc.distance, sc.bending, sc.pinned, sc.params, sc.V)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().oc_pbd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, X=None, V=None):
+        Xa = None if X is None else _vec(X, self.N)
+        Va = None if V is None else _vec(V, self.N)
+        check(lib().oc_pbd_set_state(self.h, ptr(Xa), ptr(Va)))
+
+    def get_state(self):
+        X = np.zeros((self.N, 3), np.float32); V = np.zeros((self.N, 3), np.float32)
+        check(lib().oc_pbd_get_state(self.h, ptr(X), ptr(V)))
+        return X, V
+
+    def num_colors(self):
+        a = C.c_int32(); b = C.c_int32()
+        check(lib().oc_pbd_num_colors(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def step(self, nsteps: int = 1, ell: dict | None = None):
+        """StepPhysics of the PBD demo (PBD:915-925)."""
+        e = None if ell is None else OcEllipsoid.from_dict(ell)
+        check(lib().oc_pbd_step(self.h, int(nsteps), C.byref(e) if e is not None else None))
+
+    def synchronize(self):
+        check(lib().oc_pbd_synchronize(self.h))

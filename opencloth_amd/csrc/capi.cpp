@@ -195,6 +195,7 @@ int oc_cloth_create(int32_t nx, int32_t ny, const float* x0, const float* v0,
     d.nt = d.ntx * d.nty;
     d.wa = t.wa;
     d.wp = t.wp;
+    d.persistent = (params->flags & OC_FLAG_CG_PERSISTENT) ? oc::persistent_cg_grid(d) : 0;
 
     const size_t vec = align_up(sizeof(float) * 3 * N);
     const size_t blocks = align_up(sizeof(float) * 6 * N * std::max(t.n_planes, 1));
@@ -293,6 +294,7 @@ int oc_cloth_set_params(oc_cloth c, const oc_params* p) {
     int rc = check_params(p);
     if (rc != OC_OK) return rc;
     c->prm = *p;
+    c->dc.persistent = (p->flags & OC_FLAG_CG_PERSISTENT) ? oc::persistent_cg_grid(c->dc) : 0;
     return ensure_scratch(c);
 }
 
@@ -469,11 +471,12 @@ int oc_cloth_synchronize(oc_cloth c) {
 
 int oc_cloth_last_status(oc_cloth c, int32_t* iters, int32_t* status) {
     if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
-    OC_HIP(hipMemcpyAsync(c->ctl_host, c->dc.ctl, sizeof(int) * 4, hipMemcpyDeviceToHost,
+    OC_HIP(hipMemcpyAsync(c->ctl_host, c->dc.ctl, sizeof(int) * 8, hipMemcpyDeviceToHost,
                           c->stream));
     OC_HIP(hipStreamSynchronize(c->stream));
     if (iters) *iters = c->ctl_host[1];
     if (status) *status = c->ctl_host[2];
+    if (c->ctl_host[4]) return fail(OC_ERR_DEVICE, "persistent CG grid barrier timed out");
     return OC_OK;
 }
 

@@ -26,6 +26,8 @@
 // Every CG kernel issues all loads that do not depend on the previous
 // reduction (its own vectors, pair blocks, the partials) before waiting, so
 // the cross-tile reduction overlaps the memory traffic.
+#include <algorithm>
+
 #include "oc_device.hpp"
 
 namespace oc {
@@ -168,16 +170,25 @@ __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) 
 // ---------------------------------------------------------------------------
 // CG iteration k, part 1: direction update on the tile + halo, q = S A d
 // ---------------------------------------------------------------------------
+template <int TX, int TY>
+struct SpmvShared {
+    static constexpr int WX = TX + 2 * H, WY = TY + 2 * H;
+    float wd[3][WY][WX];
+    float ws[TX * TY / 64];
+    float bc;
+};
+
+// One block's share of CG iteration k, part 1.  read_done: take the stop
+// flag of earlier launches from c.ctl[0] (multi-kernel form).  Returns true
+// when the solve has stopped (every block decides alike).
 template <int TX, int TY, int WP>
-__global__ __launch_bounds__(TX* TY) void k_cg_spmv(DevCloth c, StepScalars s, int k) {
+__device__ inline bool spmv_phase(const DevCloth& c, const StepScalars& s, int k, bool read_done,
+                                  SpmvShared<TX, TY>& sh, int vb) {
     constexpr int NT = TX * TY;
     constexpr int WX = TX + 2 * H, WY = TY + 2 * H;
     constexpr int NWIN = (WX * WY + NT - 1) / NT;
-    __shared__ float wd[3][WY][WX];
-    __shared__ float ws[NT / 64];
-    __shared__ float bc;
     const int nt = c.nt, N = c.N;
-    const int tile = tile_of_block(blockIdx.x, nt);
+    const int tile = tile_of_block(vb, nt);
     const int bx = tile % c.ntx, by = tile / c.ntx;
     const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
     const int gi = bx * TX + lx, gj = by * TY + ly;
@@ -185,7 +196,7 @@ __global__ __launch_bounds__(TX* TY) void k_cg_spmv(DevCloth c, StepScalars s, i
     const int i = gj * c.nx + gi;
 
     // ---- loads that do not depend on beta
-    const int done = c.ctl[0];
+    const int done = read_done ? c.ctl[0] : 0;
     const float delta0 = c.scal[0];
     const float delta_old = k > 0 ? c.scal[k - 1] : 0.0f;
     float pv[kPartialRegs];
@@ -230,21 +241,21 @@ __global__ __launch_bounds__(TX* TY) void k_cg_spmv(DevCloth c, StepScalars s, i
     // ---- r.r of this iteration, stopping test, beta
     if (threadIdx.x < 64) {
         const float sum = partial_finish(c.del_part + (size_t)k * nt, nt, pv);
-        if (threadIdx.x == 0) bc = sum;
+        if (threadIdx.x == 0) sh.bc = sum;
     }
     __syncthreads();
-    if (done) return;
-    const float delta = bc;
+    if (done) return true;
+    const float delta = sh.bc;
     const float thresh = s.tol_mode == 1 ? s.tol2 * (k == 0 ? delta : delta0) : s.tol2;
     const bool stop = s.tol_mode != 2 && !(delta > thresh);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (vb == 0 && threadIdx.x == 0) {
         c.scal[k] = delta;
         if (stop) {
             c.ctl[0] = 1;
             c.ctl[1] = k;
         }
     }
-    if (stop) return;
+    if (stop) return true;
     // beta = 0 once r.r has underflowed to 0 (fixed iteration counts)
     const float beta = (k > 0 && delta_old != 0.0f) ? delta / delta_old : 0.0f;
 
@@ -261,9 +272,9 @@ __global__ __launch_bounds__(TX* TY) void k_cg_spmv(DevCloth c, StepScalars s, i
                 if (wx >= H && wx < H + TX && wy >= H && wy < H + TY && gx < c.nx && gy < c.ny)
                     st3(dcur, N, gy * c.nx + gx, d);
             }
-            wd[0][wy][wx] = d.x;
-            wd[1][wy][wx] = d.y;
-            wd[2][wy][wx] = d.z;
+            sh.wd[0][wy][wx] = d.x;
+            sh.wd[1][wy][wx] = d.y;
+            sh.wd[2][wy][wx] = d.z;
         }
     }
     __syncthreads();
@@ -271,13 +282,13 @@ __global__ __launch_bounds__(TX* TY) void k_cg_spmv(DevCloth c, StepScalars s, i
     // ---- q = S A d
     float t = 0.0f;
     if (own) {
-        const V3 di = {wd[0][ly + H][lx + H], wd[1][ly + H][lx + H], wd[2][ly + H][lx + H]};
+        const V3 di = {sh.wd[0][ly + H][lx + H], sh.wd[1][ly + H][lx + H], sh.wd[2][ly + H][lx + H]};
         V3 acc = {s.mass * di.x, s.mass * di.y, s.mass * di.z};
 #pragma unroll
         for (int sl = 0; sl < WP; sl++) {
             if (m[sl] == kEmptySlot) break;
             const int wy = ly + H + slot_dj(m[sl]), wx = lx + H + slot_di(m[sl]);
-            V3 w = {wd[0][wy][wx] - di.x, wd[1][wy][wx] - di.y, wd[2][wy][wx] - di.z};
+            V3 w = {sh.wd[0][wy][wx] - di.x, sh.wd[1][wy][wx] - di.y, sh.wd[2][wy][wx] - di.z};
             V3 a = symmv(C[sl], w);
             acc = {acc.x + a.x, acc.y + a.y, acc.z + a.z};
         }
@@ -285,18 +296,25 @@ __global__ __launch_bounds__(TX* TY) void k_cg_spmv(DevCloth c, StepScalars s, i
         st3(c.q, N, i, q);
         t = dot3(di, q);
     }
-    float p = tile_sum<NT>(t, ws);
+    float p = tile_sum<NT>(t, sh.ws);
     if (threadIdx.x == 0) c.sig_part[(size_t)k * nt + tile] = p;
+    return false;
 }
 
-// CG iteration k, part 2: dv += alpha d, r -= alpha q, partial r.r
+template <int TX, int TY, int WP>
+__global__ __launch_bounds__(TX* TY) void k_cg_spmv(DevCloth c, StepScalars s, int k) {
+    __shared__ SpmvShared<TX, TY> sh;
+    spmv_phase<TX, TY, WP>(c, s, k, true, sh, blockIdx.x);
+}
+
+// CG iteration k, part 2: dv += alpha d, r -= alpha q, partial r.r.  Returns
+// true when the solve has stopped (breakdown or an earlier stop).
 template <int TX, int TY>
-__global__ __launch_bounds__(TX* TY) void k_cg_axpy(DevCloth c, StepScalars s, int k) {
+__device__ inline bool axpy_phase(const DevCloth& c, const StepScalars& s, int k, bool read_done,
+                                  float* ws, float& bc, int vb) {
     constexpr int NT = TX * TY;
-    __shared__ float ws[NT / 64];
-    __shared__ float bc;
     const int nt = c.nt, N = c.N;
-    const int tile = tile_of_block(blockIdx.x, nt);
+    const int tile = tile_of_block(vb, nt);
     const int bx = tile % c.ntx, by = tile / c.ntx;
     const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
     const int gi = bx * TX + lx, gj = by * TY + ly;
@@ -304,7 +322,7 @@ __global__ __launch_bounds__(TX* TY) void k_cg_axpy(DevCloth c, StepScalars s, i
     const int i = gj * c.nx + gi;
 
     // ---- loads that do not depend on alpha
-    const int done = c.ctl[0];
+    const int done = read_done ? c.ctl[0] : 0;
     const float delta = c.scal[k];
     float pv[kPartialRegs];
     if (threadIdx.x < 64) partial_load(c.sig_part + (size_t)k * nt, nt, pv);
@@ -321,15 +339,15 @@ __global__ __launch_bounds__(TX* TY) void k_cg_axpy(DevCloth c, StepScalars s, i
         if (threadIdx.x == 0) bc = sum;
     }
     __syncthreads();
-    if (done) return;
+    if (done) return true;
     const float sigma = bc;
     if (sigma < 0.0f || !isfinite(sigma)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (vb == 0 && threadIdx.x == 0) {
             c.ctl[0] = 1;
             c.ctl[1] = k;
             c.ctl[2] |= OC_SOLVE_BREAKDOWN;
         }
-        return;
+        return true;
     }
     // sigma == 0 only after r (hence d) underflowed: no update, no 0/0
     const float alpha = sigma == 0.0f ? 0.0f : delta / sigma;
@@ -343,6 +361,72 @@ __global__ __launch_bounds__(TX* TY) void k_cg_axpy(DevCloth c, StepScalars s, i
     }
     float p = tile_sum<NT>(t, ws);
     if (threadIdx.x == 0) c.del_part[(size_t)(k + 1) * nt + tile] = p;
+    return false;
+}
+
+template <int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_cg_axpy(DevCloth c, StepScalars s, int k) {
+    __shared__ float ws[TX * TY / 64];
+    __shared__ float bc;
+    axpy_phase<TX, TY>(c, s, k, true, ws, bc, blockIdx.x);
+}
+
+// Grid barrier of the persistent solve: every block adds 1 to *bar and waits
+// until it reaches target.  The wait is bounded; on time-out the block sets
+// ctl[4] and the caller ends the kernel, so every wave exits.
+__device__ inline bool grid_barrier(unsigned* bar, unsigned target, int* err) {
+    __syncthreads();
+    __shared__ int ok;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        int good = 1;
+        unsigned spins = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > (1u << 24)) {
+                good = 0;
+                __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+        __threadfence();
+        ok = good;
+    }
+    __syncthreads();
+    return ok != 0;
+}
+
+// All CG iterations in one cooperative launch: the spmv and axpy phases of
+// the multi-kernel form, separated by grid barriers instead of kernel
+// boundaries.  Each resident block serves tiles blockIdx.x, +gridDim.x, ...
+// with the same per-tile code.  Every block owns at least one tile and so
+// takes the same stop decisions as the others: all blocks pass the same
+// barriers.
+template <int TX, int TY, int WP>
+__global__ __launch_bounds__(TX* TY) void k_cg_persistent(DevCloth c, StepScalars s, unsigned* bar) {
+    __shared__ SpmvShared<TX, TY> sh;
+    __shared__ float ws[TX * TY / 64];
+    __shared__ float bc;
+    const unsigned nb = gridDim.x;
+    unsigned target = 0;
+    for (int k = 0; k < s.max_it; k++) {
+        bool stop = false;
+        for (int vb = blockIdx.x; vb < c.nt && !stop; vb += gridDim.x)
+            stop = spmv_phase<TX, TY, WP>(c, s, k, false, sh, vb);
+        if (stop) break;
+        target += nb;
+        if (!grid_barrier(bar, target, c.ctl + 4)) break;
+        for (int vb = blockIdx.x; vb < c.nt && !stop; vb += gridDim.x)
+            stop = axpy_phase<TX, TY>(c, s, k, false, ws, bc, vb);
+        if (stop) break;
+        target += nb;
+        if (!grid_barrier(bar, target, c.ctl + 4)) break;
+    }
+}
+
+__global__ void k_reset_barrier(unsigned* bar) {
+    if (threadIdx.x == 0) *bar = 0u;
 }
 
 // Iteration count and status of the solve (one 64-thread block).
@@ -516,8 +600,45 @@ hipError_t launch_cg_finish(const DevCloth& c, const StepScalars& s, hipStream_t
     return hipGetLastError();
 }
 
+int persistent_cg_grid(const DevCloth& c) {
+    if (c.wp > 12) return 0;
+    int dev = 0, coop = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop)
+        return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg_persistent<kTileX, kTileY, 12>,
+                                                     kTileX * kTileY, 0) != hipSuccess)
+        return 0;
+    // Every block must own at least one tile: a block without one would not
+    // see the stop decision and would wait at a barrier the others have
+    // left.  Near-equal tile counts per block; a multiple of 8 blocks keeps
+    // every tile a block serves on the block's XCD (tile_of_block).
+    const int resident = per_cu * cus;
+    if (resident < 1) return 0;
+    const int per_block = (c.nt + resident - 1) / resident;
+    int grid = (c.nt + per_block - 1) / per_block;
+    if (grid >= 8) grid = grid / 8 * 8;
+    return std::min(grid, c.nt);
+}
+
 hipError_t launch_cg(const DevCloth& c, const StepScalars& s, hipStream_t st) {
     hipError_t e;
+    if (c.persistent && s.max_it > 0) {
+        if (c.persistent > c.nt) return hipErrorInvalidValue;  // a block without a tile
+        unsigned* bar = reinterpret_cast<unsigned*>(c.ctl + 6);
+        hipLaunchKernelGGL(k_reset_barrier, dim3(1), dim3(64), 0, st, bar);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        DevCloth ca = c;
+        StepScalars sa = s;
+        void* args[] = {(void*)&ca, (void*)&sa, (void*)&bar};
+        e = hipLaunchCooperativeKernel(
+            reinterpret_cast<const void*>(k_cg_persistent<kTileX, kTileY, 12>), dim3(c.persistent),
+            dim3(kTileX * kTileY), args, 0, st);
+        if (e != hipSuccess) return e;
+        return launch_cg_finish(c, s, st);
+    }
     for (int k = 0; k < s.max_it; k++) {
         if ((e = launch_cg_spmv(c, s, k, st)) != hipSuccess) return e;
         if ((e = launch_cg_axpy(c, s, k, st)) != hipSuccess) return e;

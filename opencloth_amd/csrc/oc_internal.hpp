@@ -74,8 +74,14 @@ struct DevCloth {
     float* del_part;  // (max_it+1)*nt
     float* sig_part;  // max_it*nt
     float* scal;      // max_it+1 reduced r.r values
-    int* ctl;         // [0] done, [1] iterations, [2] status bits
+    int* ctl;         // [0] done, [1] iterations, [2] status bits, [3] sticky status,
+                      // [4] grid-barrier time-out, [6] grid-barrier counter
+    int persistent;   // > 0: run all CG iterations in one cooperative launch of
+                      // this many blocks (each serving several tiles)
 };
+
+// Blocks of the persistent CG launch (all resident at once), 0 if unsupported.
+int persistent_cg_grid(const DevCloth& c);
 
 struct StepScalars {
     float mass, gx, gy, gz, h, hh, tol2;
@@ -130,6 +136,40 @@ struct VerletScalars {
 };
 hipError_t launch_verlet_forces(const DevVerlet& c, const VerletScalars& s, hipStream_t st);
 hipError_t launch_verlet_step(const DevVerlet& c, const VerletScalars& s, hipStream_t st);
+
+// Position-based dynamics (pbd.hip).  Particle vectors are SoA planes;
+// constraints are stored colour by colour (list order inside a colour).
+constexpr int kPbdBlock = 256;     // particles per reduction block
+constexpr int kPbdSums = 16;       // reduced scalars: [0..6] pass A, [7..15] pass B
+struct DevPbd {
+    int N, nblk;
+    float* x;
+    float* v;
+    float* t;          // predicted positions tmp_X
+    const float* w;    // inverse masses
+    const int32_t* d_p1;
+    const int32_t* d_p2;
+    const float* d_rest;
+    const float* d_kp;
+    const int32_t* b_p1;
+    const int32_t* b_p2;
+    const int32_t* b_p3;
+    const float* b_rest;
+    const float* b_w;
+    const float* b_kp;
+    float* part;       // kPbdSums * nblk block partials
+    float* sums;       // kPbdSums reduced values
+};
+struct PbdScalars {
+    float mass, gx, gy, gz, dt, gd, kdamp, global_k, inv_dt;
+    int flags;
+};
+hipError_t launch_pbd_predict(const DevPbd& c, const PbdScalars& s, hipStream_t st);
+hipError_t launch_pbd_distance(const DevPbd& c, int first, int count, hipStream_t st);
+hipError_t launch_pbd_bending(const DevPbd& c, const PbdScalars& s, int first, int count,
+                              hipStream_t st);
+hipError_t launch_pbd_ground(const DevPbd& c, hipStream_t st);
+hipError_t launch_pbd_integrate(const DevPbd& c, const PbdScalars& s, hipStream_t st);
 
 // Tile shape of the implicit kernels.
 constexpr int kTileX = 32;

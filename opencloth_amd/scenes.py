@@ -152,6 +152,7 @@ class StepParams:
     cg_tol_mode: int = TOL_ABSOLUTE
     warm_start: bool = True
     ground_clamp: bool = False
+    cg_persistent: bool = False  # all CG iterations in one cooperative launch
 
 
 @dataclass
@@ -367,3 +368,94 @@ def verlet_demo_scene(n: int = 21) -> Scene:
     p = StepParams(mass=1.0, gravity=(0.0, -0.00981, 0.0), dt=1.0 / 60.0, ground_clamp=True)
     return Scene(f"verlet_demo_{n}x{n}", n, n, X, np.zeros_like(X), springs, pinned, p, 600,
                  extra={"ellipsoid": ellipsoid(), "damping": -0.0125})
+
+
+# ---------------------------------------------------------------------------
+# position-based dynamics (OpenCloth_PositionBasedDynamics/.../main.cpp, PBD)
+# ---------------------------------------------------------------------------
+# AddDistanceConstraint(a, b, k) (PBD:148-162) and the triangle form of
+# AddBendingConstraint(a, b, c, k) (PBD:164-178) as records.
+PBD_DISTANCE_DTYPE = np.dtype([("p1", "<i4"), ("p2", "<i4"), ("k", "<f4")])
+PBD_BENDING_DTYPE = np.dtype([("p1", "<i4"), ("p2", "<i4"), ("p3", "<i4"), ("k", "<f4")])
+
+
+@dataclass
+class PBDParams:
+    mass: float = 1.0 / 441.0       # per particle (PBD:114); inverse mass 1/mass, 0 if pinned
+    gravity: tuple = (0.0, -0.00981, 0.0)  # PBD:112
+    dt: float = 1.0 / 60.0          # PBD:74
+    global_damping: float = 0.98    # PBD:78
+    k_damp: float = 0.00125         # PBD:111
+    solver_iterations: int = 2      # PBD:107
+    ground: bool = True             # GroundCollision in the solver loop (PBD:887)
+
+
+@dataclass
+class PBDScene:
+    name: str
+    nx: int
+    ny: int
+    X: np.ndarray
+    V: np.ndarray
+    distance: np.ndarray  # PBD_DISTANCE_DTYPE
+    bending: np.ndarray   # PBD_BENDING_DTYPE
+    pinned: np.ndarray
+    params: PBDParams
+    steps: int
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def n_vertices(self) -> int:
+        return self.nx * self.ny
+
+
+def pbd_grid_constraints(nx: int, ny: int, k_stretch: float = 0.25, k_bend: float = 0.5):
+    """The reference's constraint lists in its order (PBD:399-433): horizontal,
+    vertical and the two shear distance constraints per quad, then the
+    triangle bending constraints along columns and along rows
+    (getIndex(i, j) = j*nx + i, PBD:306-308)."""
+    u, v = nx, ny
+    l1, l2 = np.meshgrid(np.arange(v), np.arange(u - 1), indexing="ij")
+    hor = np.stack([l1 * u + l2, l1 * u + l2 + 1], -1).reshape(-1, 2)
+    l1, l2 = np.meshgrid(np.arange(u), np.arange(v - 1), indexing="ij")
+    ver = np.stack([l2 * u + l1, (l2 + 1) * u + l1], -1).reshape(-1, 2)
+    l1, l2 = np.meshgrid(np.arange(v - 1), np.arange(u - 1), indexing="ij")
+    sh = np.stack([l1 * u + l2, (l1 + 1) * u + l2 + 1, (l1 + 1) * u + l2, l1 * u + l2 + 1], -1).reshape(-1, 2)
+    pairs = np.concatenate([hor, ver, sh])
+    dist = np.zeros(len(pairs), PBD_DISTANCE_DTYPE)
+    dist["p1"], dist["p2"], dist["k"] = pairs[:, 0], pairs[:, 1], k_stretch
+    i, j = np.meshgrid(np.arange(u), np.arange(v - 2), indexing="ij")
+    bv = np.stack([j * u + i, (j + 1) * u + i, (j + 2) * u + i], -1).reshape(-1, 3)
+    i, j = np.meshgrid(np.arange(u - 2), np.arange(v), indexing="ij")
+    bh = np.stack([j * u + i, j * u + i + 1, j * u + i + 2], -1).reshape(-1, 3)
+    tri = np.concatenate([bv, bh])
+    bend = np.zeros(len(tri), PBD_BENDING_DTYPE)
+    bend["p1"], bend["p2"], bend["p3"], bend["k"] = tri[:, 0], tri[:, 1], tri[:, 2], k_bend
+    return dist, bend
+
+
+def pbd_demo_scene(n: int = 21) -> PBDScene:
+    """The PBD demo (PBD:66-134, PBD:310-464): n x n grid at height 5 over the
+    4x4 square, kStretch 0.25, kBend 0.5, kDamp 0.00125, global damping 0.98,
+    2 solver iterations, mass 1/N, gravity -0.00981, dt 1/60, pins {0, numX},
+    the ellipsoid collider."""
+    X = grid_positions(n, n, 4.0, 5.0)
+    dist, bend = pbd_grid_constraints(n, n, 0.25, 0.5)
+    pinned = np.zeros(n * n, np.uint8)
+    pinned[0] = pinned[n - 1] = 1
+    p = PBDParams(mass=float(np.float32(1.0) / np.float32(n * n)))
+    return PBDScene(f"pbd_demo_{n}x{n}", n, n, X, np.zeros_like(X), dist, bend, pinned, p, 600,
+                    extra={"ellipsoid": ellipsoid()})
+
+
+def pbd_large_scene(n: int = 512, seed: int = 0) -> PBDScene:
+    """The PBD demo's constraints and constants on an n x n grid (mass 1/N),
+    positions perturbed by N(0, 1e-3*spacing), top edge pinned, no collider."""
+    X = grid_positions(n, n, 4.0, 5.0)
+    spacing = 4.0 / (n - 1)
+    X = (X + (1e-3 * spacing) * normal(seed, X.size).reshape(X.shape)).astype(np.float32)
+    dist, bend = pbd_grid_constraints(n, n, 0.25, 0.5)
+    pinned = np.zeros(n * n, np.uint8)
+    pinned[:n] = 1
+    p = PBDParams(mass=float(np.float32(1.0) / np.float32(n * n)))
+    return PBDScene(f"pbd_{n}x{n}", n, n, X, np.zeros_like(X), dist, bend, pinned, p, 100)

@@ -1,4 +1,4 @@
-"""ctypes front end of the CPU oracle (oracle.c).
+"""ctypes front end of the CPU oracle (oracle.c, pbd.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
 bench.py's cpu_baseline leg, never by the opencloth_amd package.
@@ -87,6 +87,16 @@ class Oracle:
         L.orc_verlet_step.argtypes = [_P, _P, _P, C.c_double, C.c_double, _P, C.c_double, C.c_int,
                                       _P, _P, _P, C.c_float]
         L.orc_step_adjoint.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
+        L.orc_ellipsoid_points.argtypes = [C.c_int, _P, _P, _P, _P, _P, C.c_float, C.c_int]
+        L.orc_pbd_create.restype = _P
+        L.orc_pbd_create.argtypes = [C.c_int, _P, _P, C.c_double, _P, C.c_int, _P, C.c_int, C.c_int]
+        L.orc_pbd_destroy.argtypes = [_P]
+        L.orc_pbd_colours.argtypes = [_P, _P, _P, _P, _P, _P, _P]
+        L.orc_pbd_rest.argtypes = [_P, _P, _P, _P, _P, _P]
+        L.orc_pbd_predict.argtypes = [_P, _P, _P, _P, _P, C.c_int, _P]
+        L.orc_pbd_project.argtypes = [_P, _P, _P, C.c_int]
+        L.orc_pbd_integrate.argtypes = [_P, _P, _P, _P, _P]
+        L.orc_pbd_step.argtypes = [_P, _P, _P, _P, C.c_int, C.c_int, _P, _P, _P, C.c_float]
 
     # -- arrays --------------------------------------------------------------
     def arr(self, a) -> np.ndarray:
@@ -110,6 +120,9 @@ class Oracle:
     # -- systems ---------------------------------------------------------------
     def system(self, n_vertices, springs, pinned=None) -> "System":
         return System(self, n_vertices, springs, pinned)
+
+    def pbd(self, X0, pinned, mass, distance, bending, solver_iterations) -> "PbdSystem":
+        return PbdSystem(self, X0, pinned, mass, distance, bending, solver_iterations)
 
 
 def make_params(p, nx=0, ny=0, tx=0, ty=0, dot_mode=0, **over) -> Params:
@@ -313,6 +326,96 @@ class System:
         for Xn, Vn, DVn in reversed(states):
             gx, gv, _, _ = self.step_adjoint(Xn, Vn, DVn, prm, gx, gv, gks)
         return gks, gx, gv
+
+
+class PbdParams(C.Structure):
+    _fields_ = [
+        ("mass", C.c_double), ("gx", C.c_double), ("gy", C.c_double), ("gz", C.c_double),
+        ("dt", C.c_double), ("global_damping", C.c_double), ("k_damp", C.c_double),
+        ("solver_iterations", C.c_int32), ("ground", C.c_int32),
+    ]
+
+
+def make_pbd_params(p, **over) -> PbdParams:
+    """From opencloth_amd.scenes.PBDParams (or anything with its fields)."""
+    q = PbdParams()
+    q.mass = p.mass
+    q.gx, q.gy, q.gz = p.gravity
+    q.dt = p.dt
+    q.global_damping = p.global_damping
+    q.k_damp = p.k_damp
+    q.solver_iterations = p.solver_iterations
+    q.ground = 1 if p.ground else 0
+    for k, v in over.items():
+        setattr(q, k, v)
+    return q
+
+
+class PbdSystem:
+    """The PBD step of pbd.c.  order 0: the reference's sequential order;
+    order 1: the HIP kernels' colour order and reduction tree."""
+
+    def __init__(self, o: Oracle, X0, pinned, mass, distance, bending, solver_iterations):
+        self.o = o
+        X0 = np.ascontiguousarray(X0, np.float32)
+        self.N = len(X0)
+        self.pins = None if pinned is None else np.ascontiguousarray(pinned, np.uint8)
+        self.dist = np.ascontiguousarray(distance)
+        self.bend = np.ascontiguousarray(bending)
+        self.nd, self.nb = len(self.dist), len(self.bend)
+        self.h = o.L.orc_pbd_create(self.N, _ptr(X0), None if self.pins is None else _ptr(self.pins),
+                                    float(mass), _ptr(self.dist), self.nd, _ptr(self.bend), self.nb,
+                                    int(solver_iterations))
+        if not self.h:
+            raise ValueError("orc_pbd_create rejected the constraints")
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self.o.L.orc_pbd_destroy(h)
+            self.h = None
+
+    def colours(self):
+        """(distance order, distance colour offsets, bending order, bending offsets)."""
+        nd, nb = C.c_int(), C.c_int()
+        self.o.L.orc_pbd_colours(self.h, C.byref(nd), C.byref(nb), None, None, None, None)
+        do = np.zeros(self.nd, np.int32); dp = np.zeros(nd.value + 1, np.int32)
+        bo = np.zeros(self.nb, np.int32); bp = np.zeros(nb.value + 1, np.int32)
+        self.o.L.orc_pbd_colours(self.h, C.byref(nd), C.byref(nb), _ptr(do), _ptr(dp), _ptr(bo), _ptr(bp))
+        return do, dp, bo, bp
+
+    def rest(self):
+        dt = self.o.dtype
+        d_rest = np.zeros(self.nd, dt); d_kp = np.zeros(self.nd, dt)
+        b_rest = np.zeros(self.nb, dt); b_w = np.zeros(self.nb, dt); b_kp = np.zeros(self.nb, dt)
+        self.o.L.orc_pbd_rest(self.h, _ptr(d_rest), _ptr(d_kp), _ptr(b_rest), _ptr(b_w), _ptr(b_kp))
+        return d_rest, d_kp, b_rest, b_w, b_kp
+
+    def predict(self, X, V, prm: PbdParams, order):
+        """Returns (V after damping, predicted positions T, angular velocity w)."""
+        o = self.o
+        X = o.arr(X); V = o.arr(V).copy()
+        T = np.zeros_like(V); w = np.zeros(3, o.dtype)
+        o.L.orc_pbd_predict(self.h, _ptr(X), _ptr(V), _ptr(T), C.byref(prm), int(order), _ptr(w))
+        return V, T, w
+
+    def project(self, T, prm: PbdParams, order):
+        T = self.o.arr(T).copy()
+        self.o.L.orc_pbd_project(self.h, _ptr(T), C.byref(prm), int(order))
+        return T
+
+    def step(self, X, V, prm: PbdParams, order, nsteps=1, ell=None):
+        o = self.o
+        X = o.arr(X).copy(); V = o.arr(V).copy()
+        if ell is None:
+            o.L.orc_pbd_step(self.h, _ptr(X), _ptr(V), C.byref(prm), int(order), int(nsteps),
+                             None, None, None, 0.0)
+        else:
+            M = np.ascontiguousarray(ell["M"], np.float32); Mi = np.ascontiguousarray(ell["Minv"], np.float32)
+            cen = np.ascontiguousarray(ell["center"], np.float32)
+            o.L.orc_pbd_step(self.h, _ptr(X), _ptr(V), C.byref(prm), int(order), int(nsteps),
+                             _ptr(M), _ptr(Mi), _ptr(cen), float(ell["radius"]))
+        return X, V
 
 
 def cg_blockdiag(o: Oracle, A, b, x0, i_max, eps, literal=False):

@@ -968,9 +968,9 @@ EXPORT int orc_step_adjoint(const orc_sys* s, const real* X, const real* V, cons
  * previous position W = X (VL:530).
  * Pinned by: tests/test_oracle_post.py (outside points unchanged, points on
  * the axis of a sphere pushed to the closed-form surface point, GLM order). */
-EXPORT void orc_ellipsoid(const orc_sys* s, real* X, real* W, const float* M, const float* Minv,
-                          const float* center, float radius, int mode) {
-    for (int i = 0; i < s->N; i++) {
+EXPORT void orc_ellipsoid_points(int N, real* X, real* W, const float* M, const float* Minv,
+                                 const float* center, float radius, int mode) {
+    for (int i = 0; i < N; i++) {
         real* x = X + 3 * i;
         real d0[3];
         for (int r = 0; r < 3; r++) {
@@ -992,6 +992,11 @@ EXPORT void orc_ellipsoid(const orc_sys* s, real* X, real* W, const float* M, co
             for (int c = 0; c < 3; c++) W[3 * i + c] = mode == 1 ? x[c] : (real)0;
         }
     }
+}
+
+EXPORT void orc_ellipsoid(const orc_sys* s, real* X, real* W, const float* M, const float* Minv,
+                          const float* center, float radius, int mode) {
+    orc_ellipsoid_points(s->N, X, W, M, Minv, center, radius, mode);
 }
 
 /* ApplyProvotDynamicInverse (IE:702-726; BW:800-823 with slack 1.01): for

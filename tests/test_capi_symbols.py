@@ -53,3 +53,26 @@ def test_create_rejects_bad_input_without_device():
     rc = lib().oc_cloth_create(3, 3, None, None, None, 0, None, ctypes.byref(p), ctypes.byref(h))
     assert rc == -1
     assert lib().oc_last_error() != b""
+
+
+def test_pbd_create_rejects_bad_constraints_without_device():
+    """oc_pbd_create checks particle indices and parameters before touching
+    the device."""
+    import numpy as np
+
+    from opencloth_amd import scenes
+    from opencloth_amd._lib import OC_ERR_INVALID_ARGUMENT, OcPbdParams, lib, ptr
+
+    X = np.zeros((4, 3), np.float32)
+    d = np.zeros(1, scenes.PBD_DISTANCE_DTYPE)
+    d["p1"], d["p2"], d["k"] = 0, 4, 0.5  # index out of range
+    b = np.zeros(0, scenes.PBD_BENDING_DTYPE)
+    p = OcPbdParams()
+    p.mass, p.dt, p.solver_iterations = 1.0, 0.01, 2
+    h = ctypes.c_void_p()
+    rc = lib().oc_pbd_create(4, ptr(X), None, ptr(d), 1, ptr(b), 0, ctypes.byref(p), ctypes.byref(h))
+    assert rc == OC_ERR_INVALID_ARGUMENT and b"distance constraint 0" in lib().oc_last_error()
+    d["p2"] = 1
+    p.solver_iterations = 0
+    rc = lib().oc_pbd_create(4, ptr(X), None, ptr(d), 1, ptr(b), 0, ctypes.byref(p), ctypes.byref(h))
+    assert rc == OC_ERR_INVALID_ARGUMENT and not h.value

@@ -155,3 +155,30 @@ def test_large_scene_two_steps(o32):
     assert it == 30 and stg == 0
     assert rel_pos_err(Xg, X) <= 1e-6
     assert np.array_equal(Xg.view(np.uint32), X.view(np.uint32))
+
+
+@pytest.mark.parametrize("scene", ["plain", "large"])
+def test_persistent_cg_matches_multi_kernel(o32, scene):
+    """OC_FLAG_CG_PERSISTENT (all CG iterations in one cooperative launch)
+    runs the same per-tile arithmetic: two steps bit-identical to the FP32
+    restatement and to the per-iteration launches."""
+    import dataclasses
+
+    from opencloth_amd.cloth import ImplicitCloth
+
+    sc = scenes.plain_scene() if scene == "plain" else scenes.large_scene()
+    say("persistent cg", scene)
+    p = dataclasses.replace(sc.params, cg_persistent=True)
+    cl = ImplicitCloth(sc.nx, sc.ny, sc.X, sc.springs, sc.pinned, p, sc.V)
+    ref = make_cloth(sc)
+    cl.step(2)
+    ref.step(2)
+    Xg, Vg = cl.get_state()
+    Xm, Vm = ref.get_state()
+    assert cl.last_status() == ref.last_status()
+    assert np.array_equal(Xg.view(np.uint32), Xm.view(np.uint32))
+    assert np.array_equal(Vg.view(np.uint32), Vm.view(np.uint32))
+    sys32 = o32.system(sc.n_vertices, sc.springs, sc.pinned)
+    X, V, _, _, iters, st = sys32.rollout(sc.X, sc.V, oracle_params(sc.params, sc.nx, sc.ny), 2)
+    assert cl.last_status() == (int(np.ravel(iters)[-1]), int(np.ravel(st)[-1]))
+    assert np.array_equal(Xg.view(np.uint32), X.view(np.uint32))
