@@ -60,6 +60,9 @@ typedef enum {
                                    launch, resident blocks serving several tiles each
                                    (same arithmetic and results, grid barriers between
                                    phases); ignored where cooperative launch is absent */
+#define OC_FLAG_JACOBI 4        /* Jacobi-preconditioned CG: Baraff and Witkin's modified PCG
+                                   with P = diag(A), the intent of BW:308-330 and BW:784-785;
+                                   the stopping tests then measure r.(P^-1 r) */
 
 /* Spring record, identical to the reference's `struct Spring` (IE:68-73). */
 typedef struct {

@@ -19,7 +19,7 @@ OC_ERR_OUT_OF_MEMORY = -4
 OC_ERR_NO_DEVICE = -5
 
 SOLVE_NOT_CONVERGED, SOLVE_BREAKDOWN, SOLVE_NONFINITE = 1, 2, 4
-OC_FLAG_GROUND_CLAMP, OC_FLAG_CG_PERSISTENT = 1, 2
+OC_FLAG_GROUND_CLAMP, OC_FLAG_CG_PERSISTENT, OC_FLAG_JACOBI = 1, 2, 4
 
 # Every symbol include/opencloth.h declares, with (restype, argtypes).
 _P = C.c_void_p

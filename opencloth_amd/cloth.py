@@ -27,8 +27,9 @@ def to_oc_params(p: StepParams) -> OcParams:
     q.cg_tol = p.cg_tol
     q.cg_tol_mode = p.cg_tol_mode
     q.warm_start = 1 if p.warm_start else 0
-    q.flags = (_lib.OC_FLAG_GROUND_CLAMP if p.ground_clamp else 0) | (
-        _lib.OC_FLAG_CG_PERSISTENT if p.cg_persistent else 0)
+    q.flags = ((_lib.OC_FLAG_GROUND_CLAMP if p.ground_clamp else 0)
+               | (_lib.OC_FLAG_CG_PERSISTENT if p.cg_persistent else 0)
+               | (_lib.OC_FLAG_JACOBI if p.cg_jacobi else 0))
     return q
 
 

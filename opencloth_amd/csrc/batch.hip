@@ -9,7 +9,7 @@
 // drained, which every workgroup reaches.
 //
 // Per cloth the arithmetic is the single-cloth step's (same per-vertex
-// gathers, pair order and expressions).  Reductions: thread t of a cloth's T
+// gathers, pair order and expressions, optional Jacobi preconditioner).  Reductions: thread t of a cloth's T
 // threads sums vertices t, t+T, ... in order, then the 64-lane wave tree,
 // then the cloth's waves in order.
 #include "oc_device.hpp"
@@ -36,8 +36,15 @@ __device__ inline float seg_sum(float t, int T, float* ws) {
     return p;
 }
 
+// Jacobi preconditioner entry 1/diag(A), 1 where the diagonal is not
+// positive and finite (as in the single-cloth kernels).
+__device__ inline float jacobi_inv(float dg) {
+    return (dg > 0.0f && isfinite(dg)) ? 1.0f / dg : 1.0f;
+}
+
 // Assembly of vertex i (global index) of a cloth with row length nx:
-// pair blocks it owns, rhs b, CG start r = d = S(b - A dv); returns r.r.
+// pair blocks it owns, rhs b, CG start r = S(b - A dv), d = z = P^-1 r (z = r
+// without the Jacobi preconditioner); returns r.z.
 __device__ inline float assemble_vertex(const DevBatch& c, const StepScalars& s, int i, int nx) {
     const int N = c.Ntot;
     const float h = s.h, hh = s.hh;
@@ -48,6 +55,7 @@ __device__ inline float assemble_vertex(const DevBatch& c, const StepScalars& s,
     V3 acc = {0.0f, 0.0f, 0.0f}, kv = {0.0f, 0.0f, 0.0f};
     if (!pinned) acc = {0.0f + s.mass * s.gx, 0.0f + s.mass * s.gy, 0.0f + s.mass * s.gz};
     V3 adv = {s.mass * dvi.x, s.mass * dvi.y, s.mass * dvi.z};
+    V3 dg = {s.mass, s.mass, s.mass};  // diag(A) = m - sum of the pair blocks' diagonals
     float C[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     for (int k = 0; k < c.wa; k++) {
         const int sp = c.ainc_s[(size_t)k * N + i];
@@ -82,6 +90,7 @@ __device__ inline float assemble_vertex(const DevBatch& c, const StepScalars& s,
             }
             V3 dvj = s.warm ? ld3(c.dv, N, j) : V3{0.0f, 0.0f, 0.0f};
             adv = add3(adv, symmv(C, sub3(dvj, dvi)));
+            dg = {dg.x - C[0], dg.y - C[1], dg.z - C[2]};
 #pragma unroll
             for (int q = 0; q < 6; q++) C[q] = 0.0f;
         }
@@ -93,8 +102,22 @@ __device__ inline float assemble_vertex(const DevBatch& c, const StepScalars& s,
     st3(c.b, N, i, b);
     V3 r = pinned ? V3{0.0f, 0.0f, 0.0f} : sub3(b, adv);
     st3(c.r, N, i, r);
+    if (s.flags & OC_FLAG_JACOBI) {
+        const V3 pv = {jacobi_inv(dg.x), jacobi_inv(dg.y), jacobi_inv(dg.z)};
+        st3(c.pinv, N, i, pv);
+        const V3 z = {pv.x * r.x, pv.y * r.y, pv.z * r.z};
+        st3(c.d, N, i, z);
+        return dot3(r, z);
+    }
     st3(c.d, N, i, r);
     return dot3(r, r);
+}
+
+// z = P^-1 r of vertex i (r itself without the Jacobi preconditioner).
+__device__ inline V3 precond(const DevBatch& c, const StepScalars& s, int i, V3 r) {
+    if (!(s.flags & OC_FLAG_JACOBI)) return r;
+    const V3 pv = ld3(c.pinv, c.Ntot, i);
+    return {pv.x * r.x, pv.y * r.y, pv.z * r.z};
 }
 
 // q_i = S (A d)_i; returns d_i . q_i
@@ -118,8 +141,8 @@ __device__ inline float spmv_vertex(const DevBatch& c, const StepScalars& s, int
     return dot3(di, q);
 }
 
-// Adjoint start for vertex i: gvt = gv' + h gx', r = d = S gvt (the solution
-// buffer, c.dv, was zeroed by the cold assembly); returns r.r.
+// Adjoint start for vertex i: gvt = gv' + h gx', r = S gvt, d = P^-1 r (the
+// solution buffer, c.dv, was zeroed by the cold assembly); returns r.(P^-1 r).
 __device__ inline float adjoint_start_vertex(const DevBatch& c, const StepScalars& s,
                                              const BatchAdjBufs& a, int i) {
     const int N = c.Ntot;
@@ -128,8 +151,9 @@ __device__ inline float adjoint_start_vertex(const DevBatch& c, const StepScalar
     st3(a.gvt, N, i, g);
     const V3 r = c.pin[i] ? V3{0.0f, 0.0f, 0.0f} : g;
     st3(c.r, N, i, r);
-    st3(c.d, N, i, r);
-    return dot3(r, r);
+    const V3 z = precond(c, s, i, r);  // the preconditioner of the cold assembly
+    st3(c.d, N, i, z);
+    return dot3(r, z);
 }
 
 // Adjoint gather for vertex i (same as the single-cloth k_adj_gather):
@@ -237,8 +261,8 @@ __global__ __launch_bounds__(NT) void k_batch_step(DevBatch c, StepScalars s, Ba
                 const float beta = sd_old[g] != 0.0f ? sd[g] / sd_old[g] : 0.0f;
                 for (int vv = lt; vv < Ni; vv += T) {
                     const int i = voff + vv;
-                    V3 r = ld3(c.r, N, i), dp = ld3(c.d, N, i);
-                    st3(c.d, N, i, V3{r.x + beta * dp.x, r.y + beta * dp.y, r.z + beta * dp.z});
+                    V3 z = precond(c, s, i, ld3(c.r, N, i)), dp = ld3(c.d, N, i);
+                    st3(c.d, N, i, V3{z.x + beta * dp.x, z.y + beta * dp.y, z.z + beta * dp.z});
                 }
             }
             __syncthreads();
@@ -273,7 +297,7 @@ __global__ __launch_bounds__(NT) void k_batch_step(DevBatch c, StepScalars s, Ba
                     r = {r.x - alpha * q.x, r.y - alpha * q.y, r.z - alpha * q.z};
                     st3(c.dv, N, i, x);
                     st3(c.r, N, i, r);
-                    t += dot3(r, r);
+                    t += dot3(r, precond(c, s, i, r));
                 }
             tot = seg_sum<NW>(t, T, ws);
             if (live && !bd && lt == 0) {

@@ -204,7 +204,7 @@ int oc_cloth_create(int32_t nx, int32_t ny, const float* x0, const float* v0,
     const size_t pair_m = align_up(sizeof(uint16_t) * std::max<size_t>(t.pair_meta.size(), 1));
     const size_t sp = align_up(sizeof(float) * S);
     const size_t pin = align_up(N);
-    const size_t total = 9 * vec + blocks + ainc_s + ainc_m + pair_m + 3 * sp + pin;
+    const size_t total = 11 * vec + blocks + ainc_s + ainc_m + pair_m + 3 * sp + pin;
     if (hipMalloc(&c->arena, total) != hipSuccess) {
         delete c;
         return fail(OC_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(total) + " bytes failed");
@@ -224,6 +224,8 @@ int oc_cloth_create(int32_t nx, int32_t ny, const float* x0, const float* v0,
     d.q = (float*)take(vec);
     d.b = (float*)take(vec);
     d.f = (float*)take(vec);
+    d.s = (float*)take(vec);
+    d.pinv = (float*)take(vec);
     d.blocks = (float*)take(blocks);
     d.ainc_s = (const int32_t*)take(ainc_s);
     d.ainc_m = (const uint16_t*)take(ainc_m);

@@ -266,7 +266,7 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
     const size_t sp = align_up(sizeof(float) * S);
     const size_t pin = align_up(N);
     const size_t per = align_up(sizeof(int) * n_cloths);
-    const size_t total = 7 * vec + blocks + as + am + pm + 3 * sp + pin + 5 * per + 256;
+    const size_t total = 8 * vec + blocks + as + am + pm + 3 * sp + pin + 5 * per + 256;
     if (hipMalloc(&b->arena, total) != hipSuccess) {
         delete b;
         return set_error(OC_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(total) + " bytes failed");
@@ -288,6 +288,7 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
     d.d = (float*)take(vec);
     d.q = (float*)take(vec);
     d.b = (float*)take(vec);
+    d.pinv = (float*)take(vec);
     d.blocks = (float*)take(blocks);
     d.ainc_s = (const int32_t*)take(as);
     d.ainc_m = (const uint16_t*)take(am);

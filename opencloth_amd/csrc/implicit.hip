@@ -44,6 +44,12 @@ __device__ inline int tile_of_block(int b, int nt) {
     return b;
 }
 
+// Jacobi preconditioner entry 1/diag(A); a non-positive or non-finite
+// diagonal (A can be indefinite under strong compression) falls back to 1.
+__device__ inline float jacobi_inv(float dg) {
+    return (dg > 0.0f && isfinite(dg)) ? 1.0f / dg : 1.0f;
+}
+
 // ---------------------------------------------------------------------------
 // forces (ComputeForces, IE:645-682): gather in spring order, pins get zero
 // ---------------------------------------------------------------------------
@@ -111,6 +117,7 @@ __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) 
         V3 acc = {0.0f, 0.0f, 0.0f}, kv = {0.0f, 0.0f, 0.0f};
         if (!pinned) acc = {0.0f + s.mass * s.gx, 0.0f + s.mass * s.gy, 0.0f + s.mass * s.gz};
         V3 adv = {s.mass * dvi.x, s.mass * dvi.y, s.mass * dvi.z};
+        V3 dg = {s.mass, s.mass, s.mass};  // diag(A) = m - sum of the pair blocks' diagonals
         float C[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int k = 0; k < WA; k++) {
@@ -149,6 +156,7 @@ __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) 
                 V3 wd = {dvj.x - dvi.x, dvj.y - dvi.y, dvj.z - dvi.z};
                 V3 a = symmv(C, wd);
                 adv = {adv.x + a.x, adv.y + a.y, adv.z + a.z};
+                dg = {dg.x - C[0], dg.y - C[1], dg.z - C[2]};
 #pragma unroll
                 for (int q = 0; q < 6; q++) C[q] = 0.0f;
             }
@@ -160,8 +168,17 @@ __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) 
         st3(c.b, N, i, b);
         V3 r = pinned ? V3{0.0f, 0.0f, 0.0f} : V3{b.x - adv.x, b.y - adv.y, b.z - adv.z};
         st3(c.r, N, i, r);
-        st3(c.d0, N, i, r);
-        t = dot3(r, r);
+        if (s.flags & OC_FLAG_JACOBI) {
+            const V3 pv = {jacobi_inv(dg.x), jacobi_inv(dg.y), jacobi_inv(dg.z)};
+            st3(c.pinv, N, i, pv);
+            const V3 z = {pv.x * r.x, pv.y * r.y, pv.z * r.z};
+            st3(c.s, N, i, z);
+            st3(c.d0, N, i, z);
+            t = dot3(r, z);
+        } else {
+            st3(c.d0, N, i, r);
+            t = dot3(r, r);
+        }
     }
     float p = tile_sum<NT>(t, ws);
     if (threadIdx.x == 0) c.del_part[tile] = p;
@@ -203,6 +220,7 @@ __device__ inline bool spmv_phase(const DevCloth& c, const StepScalars& s, int k
     if (threadIdx.x < 64) partial_load(c.del_part + (size_t)k * nt, nt, pv);
     const float* dprev = (k & 1) ? c.d0 : c.d1;
     float* dcur = (k & 1) ? c.d1 : c.d0;
+    const float* zsrc = (s.flags & OC_FLAG_JACOBI) ? c.s : c.r;  // d = z + beta d
     const int x0 = bx * TX - H, y0 = by * TY - H;
     V3 wr[NWIN], wp[NWIN];
 #pragma unroll
@@ -215,7 +233,7 @@ __device__ inline bool spmv_phase(const DevCloth& c, const StepScalars& s, int k
             if (k == 0) {
                 wr[n] = ld3(c.d0, N, g);
             } else {
-                wr[n] = ld3(c.r, N, g);
+                wr[n] = ld3(zsrc, N, g);
                 wp[n] = ld3(dprev, N, g);
             }
         }
@@ -327,12 +345,14 @@ __device__ inline bool axpy_phase(const DevCloth& c, const StepScalars& s, int k
     float pv[kPartialRegs];
     if (threadIdx.x < 64) partial_load(c.sig_part + (size_t)k * nt, nt, pv);
     const float* dcur = (k & 1) ? c.d1 : c.d0;
-    V3 d = {0.0f, 0.0f, 0.0f}, q = d, x = d, r = d;
+    const bool jac = (s.flags & OC_FLAG_JACOBI) != 0;
+    V3 d = {0.0f, 0.0f, 0.0f}, q = d, x = d, r = d, jv = d;
     if (own) {
         d = ld3(dcur, N, i);
         q = ld3(c.q, N, i);
         x = ld3(c.dv, N, i);
         r = ld3(c.r, N, i);
+        if (jac) jv = ld3(c.pinv, N, i);
     }
     if (threadIdx.x < 64) {
         const float sum = partial_finish(c.sig_part + (size_t)k * nt, nt, pv);
@@ -357,7 +377,13 @@ __device__ inline bool axpy_phase(const DevCloth& c, const StepScalars& s, int k
         r = {r.x - alpha * q.x, r.y - alpha * q.y, r.z - alpha * q.z};
         st3(c.dv, N, i, x);
         st3(c.r, N, i, r);
-        t = dot3(r, r);
+        if (jac) {
+            const V3 z = {jv.x * r.x, jv.y * r.y, jv.z * r.z};
+            st3(c.s, N, i, z);
+            t = dot3(r, z);
+        } else {
+            t = dot3(r, r);
+        }
     }
     float p = tile_sum<NT>(t, ws);
     if (threadIdx.x == 0) c.del_part[(size_t)(k + 1) * nt + tile] = p;
@@ -499,9 +525,17 @@ __global__ __launch_bounds__(TX* TY) void k_adj_init(DevCloth c, StepScalars s, 
         st3(a.gvt, N, i, g);
         V3 r = c.pin[i] ? V3{0.0f, 0.0f, 0.0f} : g;
         st3(c.r, N, i, r);
-        st3(c.d0, N, i, r);
         st3(c.dv, N, i, V3{0.0f, 0.0f, 0.0f});
-        t = dot3(r, r);
+        if (s.flags & OC_FLAG_JACOBI) {  // pinv from the re-assembly
+            const V3 pv = ld3(c.pinv, N, i);
+            const V3 z = {pv.x * r.x, pv.y * r.y, pv.z * r.z};
+            st3(c.s, N, i, z);
+            st3(c.d0, N, i, z);
+            t = dot3(r, z);
+        } else {
+            st3(c.d0, N, i, r);
+            t = dot3(r, r);
+        }
     }
     float p = tile_sum<NT>(t, ws);
     if (threadIdx.x == 0) c.del_part[tile] = p;

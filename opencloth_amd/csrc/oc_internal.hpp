@@ -63,6 +63,8 @@ struct DevCloth {
     float* q;
     float* b;
     float* f;
+    float* s;       // z = P^-1 r of the Jacobi-preconditioned solve
+    float* pinv;    // Jacobi preconditioner 1/diag(A)
     const uint8_t* pin;
     float* blocks;  // n_planes*6*N
     const int32_t* ainc_s;
@@ -189,6 +191,7 @@ struct DevBatch {
     float* d;
     float* q;
     float* b;
+    float* pinv;    // Jacobi preconditioner 1/diag(A)
     const uint8_t* pin;
     float* blocks;  // n_planes*6*Ntot
     const int32_t* ainc_s;

@@ -153,6 +153,7 @@ class StepParams:
     warm_start: bool = True
     ground_clamp: bool = False
     cg_persistent: bool = False  # all CG iterations in one cooperative launch
+    cg_jacobi: bool = False      # Jacobi-preconditioned CG (Baraff-Witkin modified PCG)
 
 
 @dataclass

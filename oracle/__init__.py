@@ -19,6 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 REF = os.path.join(HERE, "_ref")
 
 NOT_CONVERGED, BREAKDOWN, NONFINITE = 1, 2, 4
+JACOBI = 4  # Params.flags bit: Jacobi-preconditioned CG
 
 
 class Params(C.Structure):
@@ -138,7 +139,8 @@ def make_params(p, nx=0, ny=0, tx=0, ty=0, dot_mode=0, **over) -> Params:
     q.warm_start = int(bool(over.get("warm_start", p.warm_start)))
     q.dot_mode = dot_mode
     q.nx, q.ny, q.tx, q.ty = nx, ny, tx, ty
-    q.flags = 1 if over.get("ground_clamp", p.ground_clamp) else 0
+    q.flags = (1 if over.get("ground_clamp", p.ground_clamp) else 0) | (
+        JACOBI if over.get("cg_jacobi", getattr(p, "cg_jacobi", False)) else 0)
     return q
 
 

@@ -80,6 +80,10 @@ typedef struct {
 #define ORC_BREAKDOWN 2
 #define ORC_NONFINITE 4
 
+/* orc_params.flags */
+#define ORC_FLAG_GROUND_CLAMP 1
+#define ORC_FLAG_JACOBI 4 /* Jacobi-preconditioned CG (Baraff-Witkin modified PCG) */
+
 /* ------------------------------------------------------------------------ */
 /* vec3 / symmetric-block helpers in GLM evaluation order                    */
 /* ------------------------------------------------------------------------ */
@@ -645,26 +649,36 @@ EXPORT double orc_dot(const orc_sys* s, const orc_params* prm, const real* a, co
  * so a fixed iteration count never divides 0 by 0).
  * The recurrence is written once (cg_core) over a matrix-vector callback so
  * that the same code also runs on block-diagonal systems.
+ * With a Jacobi inverse pinv (flags ORC_FLAG_JACOBI) it is Baraff and
+ * Witkin's modified PCG (1998, sec. 5.3), the intent of BW:308-330 and
+ * BW:784-785: z = P^-1 r, delta = r.z (also for the stopping tests),
+ * d = z + beta d; the filter keeps pinned entries of r, z and d at zero.
+ * (The reference's version takes delta0 = r.(P r) and drops P after the
+ * first iteration; that is not restated.)
  * Returns iterations run; *status gets ORC_* bits.
  * Pinned by: a dense FP64 solve of S A S on tiny grids; bitwise equality of
  * cg_core on block-diagonal systems with a transcription of IE:227-249
- * (orc_cg_reference_blockdiag); the residual recurrence r_k = S(b - A x_k). */
+ * (orc_cg_reference_blockdiag); the residual recurrence r_k = S(b - A x_k);
+ * for the preconditioned form, the dense FP64 solve and a dense
+ * preconditioned-CG transcription in the tests. */
 typedef void (*matvec_fn)(const void* ctx, const real* in, real* out);
 typedef real (*dot_fn)(const void* ctx, const real* a, const real* b);
 
 static int cg_core(int N, const uint8_t* pinned, matvec_fn mv, dot_fn dot, const void* ctx,
-                   int max_it, real tol, int tol_mode, const real* b, real* dv, int* status,
-                   double* out_delta) {
+                   int max_it, real tol, int tol_mode, const real* pinv, const real* b, real* dv,
+                   int* status, double* out_delta) {
     real* r = (real*)malloc(sizeof(real) * 3 * (N > 0 ? N : 1));
     real* d = (real*)malloc(sizeof(real) * 3 * (N > 0 ? N : 1));
     real* q = (real*)malloc(sizeof(real) * 3 * (N > 0 ? N : 1));
+    real* z = pinv ? (real*)malloc(sizeof(real) * 3 * (N > 0 ? N : 1)) : r; /* P^-1 r */
     mv(ctx, dv, q);
     for (int i = 0; i < 3 * N; i++) {
         r[i] = (pinned && pinned[i / 3]) ? (real)0 : b[i] - q[i];
-        d[i] = r[i];
+        if (pinv) z[i] = pinv[i] * r[i];
+        d[i] = z[i];
     }
     real tol2 = tol * tol;
-    real delta = dot(ctx, r, r);
+    real delta = dot(ctx, r, z);
     real delta0 = delta;
     real thresh = tol_mode == 1 ? tol2 * delta0 : tol2;
     int it = 0, st = 0;
@@ -682,17 +696,19 @@ static int cg_core(int N, const uint8_t* pinned, matvec_fn mv, dot_fn dot, const
         for (int i = 0; i < 3 * N; i++) {
             dv[i] = dv[i] + alpha * d[i];
             r[i] = r[i] - alpha * q[i];
+            if (pinv) z[i] = pinv[i] * r[i];
         }
         real delta_old = delta;
-        delta = dot(ctx, r, r);
+        delta = dot(ctx, r, z);
         real beta = delta_old == 0 ? (real)0 : delta / delta_old;
-        for (int i = 0; i < 3 * N; i++) d[i] = r[i] + beta * d[i];
+        for (int i = 0; i < 3 * N; i++) d[i] = z[i] + beta * d[i];
         it++;
     }
     if (tol_mode != 2 && delta > thresh && !(st & ORC_BREAKDOWN)) st |= ORC_NOT_CONVERGED;
     if (!ISFINITE_(delta)) st |= ORC_NONFINITE;
     if (status) *status = st;
     if (out_delta) *out_delta = (double)delta;
+    if (pinv) free(z);
     free(r);
     free(d);
     free(q);
@@ -714,11 +730,33 @@ static real pair_dot(const void* ctx, const real* a, const real* b) {
     return dot_vec(c->s, c->prm, a, b);
 }
 
+/* Jacobi preconditioner (flags bit ORC_FLAG_JACOBI): P^-1 = 1/diag(A) per
+ * component, diag(A)_i = m - sum of the pair blocks' diagonals in the
+ * vertex's pair order; a non-positive or non-finite diagonal entry (possible
+ * under strong compression, where A can be indefinite) falls back to 1.
+ * Returns NULL when the flag is off (plain CG). */
+static real* jacobi_inverse(const orc_sys* s, const real* Cp, const orc_params* prm) {
+    if (!(prm->flags & ORC_FLAG_JACOBI)) return NULL;
+    real* pinv = (real*)malloc(sizeof(real) * 3 * (s->N > 0 ? s->N : 1));
+    real m = (real)prm->mass;
+    for (int i = 0; i < s->N; i++) {
+        real dg[3] = {m, m, m};
+        for (int k = s->vp_ptr[i]; k < s->vp_ptr[i + 1]; k++)
+            for (int c = 0; c < 3; c++) dg[c] -= Cp[6 * s->vp_pair[k] + c];
+        for (int c = 0; c < 3; c++)
+            pinv[3 * i + c] = (dg[c] > 0 && ISFINITE_(dg[c])) ? (real)1 / dg[c] : (real)1;
+    }
+    return pinv;
+}
+
 EXPORT int orc_cg(const orc_sys* s, const real* Cp, const orc_params* prm, const real* b, real* dv,
                   int* status, double* out_delta) {
     pair_ctx c = {s, Cp, prm};
-    return cg_core(s->N, s->pinned, pair_mv, pair_dot, &c, prm->max_it, (real)prm->tol,
-                   prm->tol_mode, b, dv, status, out_delta);
+    real* pinv = jacobi_inverse(s, Cp, prm);
+    int it = cg_core(s->N, s->pinned, pair_mv, pair_dot, &c, prm->max_it, (real)prm->tol,
+                     prm->tol_mode, pinv, b, dv, status, out_delta);
+    free(pinv);
+    return it;
 }
 
 typedef struct {
@@ -746,7 +784,7 @@ static real bd_dot(const void* ctx, const real* a, const real* b) {
  * stopping test. */
 EXPORT int orc_cg_blockdiag(int N, const real* A, const real* b, real* x, int i_max, double eps) {
     bd_ctx c = {N, A};
-    return cg_core(N, NULL, bd_mv, bd_dot, &c, i_max, (real)eps, 0, b, x, NULL, NULL);
+    return cg_core(N, NULL, bd_mv, bd_dot, &c, i_max, (real)eps, 0, NULL, b, x, NULL, NULL);
 }
 
 /* Line-by-line transcription of SolveConjugateGradient (IE:227-249) with the
@@ -917,8 +955,10 @@ EXPORT int orc_step_adjoint(const orc_sys* s, const real* X, const real* V, cons
     }
     orc_assemble(s, X, V, prm, B, Cp, NULL);
     pair_ctx pc = {s, Cp, prm};
+    real* pinv = jacobi_inverse(s, Cp, prm);
     int it = cg_core(N, s->pinned, pair_mv, pair_dot, &pc, prm->max_it, (real)prm->tol,
-                     prm->tol_mode, gvt, mu, status, NULL);
+                     prm->tol_mode, pinv, gvt, mu, status, NULL);
+    free(pinv);
     for (int i = 0; i < N; i++) {
         real gx[3] = {gx_out[3 * i], gx_out[3 * i + 1], gx_out[3 * i + 2]};
         real gv[3] = {gvt[3 * i], gvt[3 * i + 1], gvt[3 * i + 2]};
