@@ -94,6 +94,12 @@ const char* oc_last_error(void);
 /* Number of visible HIP devices (0 when none). */
 int oc_device_count(void);
 
+/* Render triangles of an nx*ny grid: 2*(nx-1)*(ny-1) triangles, 3 vertex
+ * indices each, written to out (host memory, 6*(nx-1)*(ny-1) int32), in the
+ * reference's order and winding (IE:473-488) but 32-bit (the reference's
+ * GLushort indices overflow past 65,535 vertices, IE:276).  Host only. */
+int oc_grid_triangles(int32_t nx, int32_t ny, int32_t* out);
+
 /* ---- implicit cloth (the hot path) ------------------------------------- */
 typedef struct oc_cloth_s* oc_cloth;
 

@@ -29,6 +29,7 @@ SIGNATURES = {
     "oc_version": (C.c_char_p, []),
     "oc_last_error": (C.c_char_p, []),
     "oc_device_count": (C.c_int, []),
+    "oc_grid_triangles": (C.c_int, [_I, _I, _P]),
     "oc_cloth_create": (C.c_int, [_I, _I, _P, _P, _P, _I, _P, _P, _P]),
     "oc_cloth_destroy": (C.c_int, [_P]),
     "oc_cloth_set_stream": (C.c_int, [_P, _P]),

@@ -160,6 +160,20 @@ int oc_device_count(void) {
     return n;
 }
 
+int oc_grid_triangles(int32_t nx, int32_t ny, int32_t* out) {
+    if (!out) return fail(OC_ERR_INVALID_ARGUMENT, "out is NULL");
+    if (nx < 2 || ny < 2) return fail(OC_ERR_INVALID_ARGUMENT, "grid must be at least 2x2");
+    int32_t* o = out;
+    for (int i = 0; i < ny - 1; i++)
+        for (int j = 0; j < nx - 1; j++) {
+            const int32_t i0 = i * nx + j, i1 = i0 + 1, i2 = i0 + nx, i3 = i2 + 1;
+            const int32_t odd[6] = {i0, i2, i1, i1, i2, i3}, even[6] = {i0, i2, i3, i0, i3, i1};
+            const int32_t* t = ((i + j) & 1) ? odd : even;
+            for (int k = 0; k < 6; k++) *o++ = t[k];
+        }
+    return OC_OK;
+}
+
 int oc_cloth_create(int32_t nx, int32_t ny, const float* x0, const float* v0,
                     const oc_spring* springs, int32_t n_springs, const uint8_t* pinned,
                     const oc_params* params, oc_cloth* out) {
