@@ -42,6 +42,32 @@ __device__ inline float jacobi_inv(float dg) {
     return (dg > 0.0f && isfinite(dg)) ? 1.0f / dg : 1.0f;
 }
 
+// Slot k of vertex i: spring id and meta, the other end's x, v (dv for a
+// warm start) and the spring's constants; sp < 0 past the last slot.
+struct BSpringIn {
+    int sp;
+    uint16_t m;
+    V3 x, v, dv;
+    float L0, ks, kd;
+};
+
+__device__ inline BSpringIn fetch_bspring(const DevBatch& c, const StepScalars& s, int i, int nx, int k) {
+    BSpringIn r{};
+    const int N = c.Ntot;
+    r.sp = c.ainc_s[(size_t)k * N + i];
+    if (r.sp >= 0) {
+        r.m = c.ainc_m[(size_t)k * N + i];
+        const int j = i + slot_dj(r.m) * nx + slot_di(r.m);
+        r.x = ld3(c.x, N, j);
+        r.v = ld3(c.v, N, j);
+        r.dv = s.warm ? ld3(c.dv, N, j) : V3{0.0f, 0.0f, 0.0f};
+        r.L0 = c.L0[r.sp];
+        r.ks = c.ks[r.sp];
+        r.kd = c.kd[r.sp];
+    }
+    return r;
+}
+
 // Assembly of vertex i (global index) of a cloth with row length nx:
 // pair blocks it owns, rhs b, CG start r = S(b - A dv), d = z = P^-1 r (z = r
 // without the Jacobi preconditioner); returns r.z.
@@ -57,39 +83,31 @@ __device__ inline float assemble_vertex(const DevBatch& c, const StepScalars& s,
     V3 adv = {s.mass * dvi.x, s.mass * dvi.y, s.mass * dvi.z};
     V3 dg = {s.mass, s.mass, s.mass};  // diag(A) = m - sum of the pair blocks' diagonals
     float C[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    // Springs are evaluated from this vertex's side (x1 = x_i), which is
+    // bitwise the negated p1-side force and K w; the next spring's data is
+    // loaded before the current one is evaluated.
+    BSpringIn nxt{};
+    nxt.sp = -1;
+    if (c.wa > 0) nxt = fetch_bspring(c, s, i, nx, 0);
     for (int k = 0; k < c.wa; k++) {
-        const int sp = c.ainc_s[(size_t)k * N + i];
-        if (sp < 0) break;
-        const uint16_t m = c.ainc_m[(size_t)k * N + i];
-        const int j = i + slot_dj(m) * nx + slot_di(m);
-        V3 xj = ld3(c.x, N, j), vj = ld3(c.v, N, j);
-        const bool second = (m >> 8) & 1u;
+        const BSpringIn in = nxt;
+        if (in.sp < 0) break;
+        if (k + 1 < c.wa) nxt = fetch_bspring(c, s, i, nx, k + 1);
+        const uint16_t m = in.m;
         V3 f;
         float K[6], P[6];
-        const float kd = c.kd[sp];
-        if (second)
-            spring_terms(xj, xi, vj, vi, c.L0[sp], c.ks[sp], kd, f, K, P);
-        else
-            spring_terms(xi, xj, vi, vj, c.L0[sp], c.ks[sp], kd, f, K, P);
+        spring_terms(xi, in.x, vi, in.v, in.L0, in.ks, in.kd, f, K, P);
 #pragma unroll
-        for (int q = 0; q < 6; q++) C[q] += h * (kd * P[q]) + hh * K[q];
-        V3 w = second ? sub3(vj, vi) : sub3(vi, vj);
-        V3 kw = symmv(K, w);
-        if (second) {
-            acc = sub3(acc, f);
-            kv = sub3(kv, kw);
-        } else {
-            acc = add3(acc, f);
-            kv = add3(kv, kw);
-        }
+        for (int q = 0; q < 6; q++) C[q] += h * (in.kd * P[q]) + hh * K[q];
+        acc = add3(acc, f);
+        kv = add3(kv, symmv(K, sub3(vi, in.v)));
         if ((m >> 13) & 1u) {
             if ((m >> 14) & 1u) {
                 const int plane = (m >> 9) & 15;
 #pragma unroll
                 for (int q = 0; q < 6; q++) c.blocks[(size_t)(plane * 6 + q) * N + i] = C[q];
             }
-            V3 dvj = s.warm ? ld3(c.dv, N, j) : V3{0.0f, 0.0f, 0.0f};
-            adv = add3(adv, symmv(C, sub3(dvj, dvi)));
+            adv = add3(adv, symmv(C, sub3(in.dv, dvi)));
             dg = {dg.x - C[0], dg.y - C[1], dg.z - C[2]};
 #pragma unroll
             for (int q = 0; q < 6; q++) C[q] = 0.0f;
@@ -120,21 +138,44 @@ __device__ inline V3 precond(const DevBatch& c, const StepScalars& s, int i, V3 
     return {pv.x * r.x, pv.y * r.y, pv.z * r.z};
 }
 
-// q_i = S (A d)_i; returns d_i . q_i
+// Pair slot data for the SpMV: meta, the block (6 floats) and d of the
+// other end.
+struct PairIn {
+    uint16_t m;
+    float C[6];
+    V3 dj;
+};
+
+__device__ inline PairIn fetch_pair(const DevBatch& c, int i, int nx, uint16_t m) {
+    PairIn r{};
+    r.m = m;
+    if (m != kEmptySlot) {
+        const int N = c.Ntot;
+        const int j = i + slot_dj(m) * nx + slot_di(m);
+        const int blk = ((m >> 12) & 1) ? i : j;
+        const float* bp = c.blocks + (size_t)((m >> 8) & 15) * 6 * N + blk;
+#pragma unroll
+        for (int q = 0; q < 6; q++) r.C[q] = bp[(size_t)q * N];
+        r.dj = ld3(c.d, N, j);
+    }
+    return r;
+}
+
+// q_i = S (A d)_i; returns d_i . q_i.  Pair metas are loaded two slots
+// ahead and block data one slot ahead of the slot being summed.
 __device__ inline float spmv_vertex(const DevBatch& c, const StepScalars& s, int i, int nx) {
     const int N = c.Ntot;
     const V3 di = ld3(c.d, N, i);
     V3 acc = scl3(s.mass, di);
+    const uint16_t m0 = c.wp > 0 ? c.pair_m[i] : kEmptySlot;
+    uint16_t m1 = c.wp > 1 ? c.pair_m[(size_t)N + i] : kEmptySlot;
+    PairIn cur = fetch_pair(c, i, nx, m0);
     for (int sl = 0; sl < c.wp; sl++) {
-        const uint16_t m = c.pair_m[(size_t)sl * N + i];
-        if (m == kEmptySlot) break;
-        const int j = i + slot_dj(m) * nx + slot_di(m);
-        const int blk = ((m >> 12) & 1) ? i : j;
-        const float* bp = c.blocks + (size_t)((m >> 8) & 15) * 6 * N + blk;
-        float C[6];
-#pragma unroll
-        for (int q = 0; q < 6; q++) C[q] = bp[(size_t)q * N];
-        acc = add3(acc, symmv(C, sub3(ld3(c.d, N, j), di)));
+        if (cur.m == kEmptySlot) break;
+        const PairIn nxt = fetch_pair(c, i, nx, m1);
+        m1 = (sl + 2 < c.wp) ? c.pair_m[(size_t)(sl + 2) * N + i] : kEmptySlot;
+        acc = add3(acc, symmv(cur.C, sub3(cur.dj, di)));
+        cur = nxt;
     }
     const V3 q = c.pin[i] ? V3{0.0f, 0.0f, 0.0f} : acc;
     st3(c.q, N, i, q);

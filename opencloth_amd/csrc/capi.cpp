@@ -389,8 +389,8 @@ int oc_cloth_integrate_implicit(oc_cloth c, float dt) {
     if (!(dt > 0.0f)) return fail(OC_ERR_INVALID_ARGUMENT, "dt must be positive");
     const oc::StepScalars s = scalars(c, dt);
     OC_HIP(oc::launch_assemble(c->dc, s, c->stream));
-    OC_HIP(oc::launch_cg(c->dc, s, c->stream));
-    OC_HIP(oc::launch_update(c->dc, s, c->stream));
+    OC_HIP(oc::launch_cg(c->dc, s, c->stream, false));
+    OC_HIP(oc::launch_update(c->dc, s, c->stream, true));
     c->assembled = false;
     return OC_OK;
 }
@@ -401,8 +401,8 @@ int oc_cloth_step(oc_cloth c, int32_t nsteps) {
     const oc::StepScalars s = scalars(c, c->prm.dt);
     for (int k = 0; k < nsteps; k++) {
         OC_HIP(oc::launch_assemble(c->dc, s, c->stream));
-        OC_HIP(oc::launch_cg(c->dc, s, c->stream));
-        OC_HIP(oc::launch_update(c->dc, s, c->stream));
+        OC_HIP(oc::launch_cg(c->dc, s, c->stream, false));
+        OC_HIP(oc::launch_update(c->dc, s, c->stream, true));
     }
     c->assembled = false;
     return OC_OK;
@@ -412,7 +412,7 @@ int oc_cloth_profile_steps(oc_cloth c, int32_t nsteps, float* ms, int32_t* launc
     if (!c || !ms || !launches) return fail(OC_ERR_INVALID_ARGUMENT, "NULL argument");
     if (nsteps < 0) return fail(OC_ERR_INVALID_ARGUMENT, "nsteps < 0");
     const oc::StepScalars s = scalars(c, c->prm.dt);
-    const int per_step = 2 * s.max_it + 3;
+    const int per_step = 2 * s.max_it + 2;
     std::vector<hipEvent_t> ev(per_step + 1);
     std::vector<int> cls(per_step);
     for (auto& e : ev) OC_HIP(hipEventCreate(&e));
@@ -433,8 +433,7 @@ int oc_cloth_profile_steps(oc_cloth c, int32_t nsteps, float* ms, int32_t* launc
         for (int k = 0; ok && k < s.max_it; k++)
             ok = rec(oc::launch_cg_spmv(c->dc, s, k, c->stream), 1) &&
                  rec(oc::launch_cg_axpy(c->dc, s, k, c->stream), 2);
-        ok = ok && rec(oc::launch_cg_finish(c->dc, s, c->stream), 3) &&
-             rec(oc::launch_update(c->dc, s, c->stream), 4);
+        ok = ok && rec(oc::launch_update(c->dc, s, c->stream, true), 4);
         if (!ok || hipEventSynchronize(ev[n]) != hipSuccess) {
             rc = fail(OC_ERR_DEVICE, "profiled launch failed");
             break;
@@ -470,8 +469,8 @@ int oc_cloth_step_physics(oc_cloth c, int32_t nsteps, const oc_ellipsoid* e, flo
     const oc::StepScalars s = scalars(c, c->prm.dt);
     for (int k = 0; k < nsteps; k++) {
         OC_HIP(oc::launch_assemble(c->dc, s, c->stream));
-        OC_HIP(oc::launch_cg(c->dc, s, c->stream));
-        OC_HIP(oc::launch_update(c->dc, s, c->stream));
+        OC_HIP(oc::launch_cg(c->dc, s, c->stream, false));
+        OC_HIP(oc::launch_update(c->dc, s, c->stream, true));
         if (e) OC_HIP(oc::launch_ellipsoid(c->dc.x, c->dc.v, c->dc.N, *e, 0, c->stream));
         if (slack > 0.0f) OC_HIP(oc::launch_provot(c->dc, slack, c->stream));
     }
@@ -592,8 +591,8 @@ int load_state(oc_cloth_s* c, const float* src) {
 
 int one_step(oc_cloth_s* c, const oc::StepScalars& s) {
     OC_HIP(oc::launch_assemble(c->dc, s, c->stream));
-    OC_HIP(oc::launch_cg(c->dc, s, c->stream));
-    OC_HIP(oc::launch_update(c->dc, s, c->stream));
+    OC_HIP(oc::launch_cg(c->dc, s, c->stream, false));
+    OC_HIP(oc::launch_update(c->dc, s, c->stream, true));
     return OC_OK;
 }
 

@@ -86,6 +86,31 @@ __global__ __launch_bounds__(TX* TY) void k_forces(DevCloth c, StepScalars s) {
 // ---------------------------------------------------------------------------
 // assembly + CG start
 // ---------------------------------------------------------------------------
+// Neighbour data of one incident spring: the other end's x, v (and dv for a
+// warm start) and the spring's constants.
+struct SpringIn {
+    V3 x, v, dv;
+    float L0, ks, kd;
+};
+
+template <int WA>
+__device__ inline SpringIn fetch_spring(const DevCloth& c, const StepScalars& s, const int (&sps)[WA],
+                                        const uint16_t (&ms)[WA], int i, int k) {
+    SpringIn r{};
+    const int sp = sps[k];
+    if (sp >= 0) {
+        const int N = c.N;
+        const int j = i + slot_dj(ms[k]) * c.nx + slot_di(ms[k]);
+        r.x = ld3(c.x, N, j);
+        r.v = ld3(c.v, N, j);
+        r.dv = s.warm ? ld3(c.dv, N, j) : V3{0.0f, 0.0f, 0.0f};
+        r.L0 = c.L0[sp];
+        r.ks = c.ks[sp];
+        r.kd = c.kd[sp];
+    }
+    return r;
+}
+
 template <int TX, int TY, int WA>
 __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) {
     constexpr int NT = TX * TY;
@@ -119,41 +144,39 @@ __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) 
         V3 adv = {s.mass * dvi.x, s.mass * dvi.y, s.mass * dvi.z};
         V3 dg = {s.mass, s.mass, s.mass};  // diag(A) = m - sum of the pair blocks' diagonals
         float C[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        // Every spring is evaluated from this vertex's side (x1 = x_i): when
+        // i is the spring's p2 that negates dp, f and K(v_i - v_j) exactly
+        // and leaves K and P unchanged, so adding them equals subtracting
+        // the p1-side values bit for bit.  The neighbour data of the next
+        // kPrefetch springs is loaded ahead of the one being evaluated.
+        constexpr int kPrefetch = 2;
+        SpringIn pre[kPrefetch];
+#pragma unroll
+        for (int k = 0; k < kPrefetch; k++) pre[k] = fetch_spring(c, s, sps, ms, i, k);
 #pragma unroll
         for (int k = 0; k < WA; k++) {
             const int sp = sps[k];
             if (sp < 0) break;
             const uint16_t m = ms[k];
-            const int j = i + slot_dj(m) * c.nx + slot_di(m);
-            V3 xj = ld3(c.x, N, j), vj = ld3(c.v, N, j);
-            const bool second = (m >> 8) & 1u;
+            const SpringIn in = pre[0];
+#pragma unroll
+            for (int q = 0; q + 1 < kPrefetch; q++) pre[q] = pre[q + 1];
+            if (k + kPrefetch < WA) pre[kPrefetch - 1] = fetch_spring(c, s, sps, ms, i, k + kPrefetch);
             V3 f;
             float K[6], P[6];
-            const float kd = c.kd[sp];
-            if (second)
-                spring_terms(xj, xi, vj, vi, c.L0[sp], c.ks[sp], kd, f, K, P);
-            else
-                spring_terms(xi, xj, vi, vj, c.L0[sp], c.ks[sp], kd, f, K, P);
+            spring_terms(xi, in.x, vi, in.v, in.L0, in.ks, in.kd, f, K, P);
 #pragma unroll
-            for (int q = 0; q < 6; q++) C[q] += h * (kd * P[q]) + hh * K[q];
-            V3 w = second ? V3{vj.x - vi.x, vj.y - vi.y, vj.z - vi.z}
-                          : V3{vi.x - vj.x, vi.y - vj.y, vi.z - vj.z};
-            V3 kw = symmv(K, w);
-            if (second) {
-                acc = {acc.x - f.x, acc.y - f.y, acc.z - f.z};
-                kv = {kv.x - kw.x, kv.y - kw.y, kv.z - kw.z};
-            } else {
-                acc = {acc.x + f.x, acc.y + f.y, acc.z + f.z};
-                kv = {kv.x + kw.x, kv.y + kw.y, kv.z + kw.z};
-            }
+            for (int q = 0; q < 6; q++) C[q] += h * (in.kd * P[q]) + hh * K[q];
+            const V3 kw = symmv(K, V3{vi.x - in.v.x, vi.y - in.v.y, vi.z - in.v.z});
+            acc = {acc.x + f.x, acc.y + f.y, acc.z + f.z};
+            kv = {kv.x + kw.x, kv.y + kw.y, kv.z + kw.z};
             if ((m >> 13) & 1u) {  // last spring of this pair
                 if ((m >> 14) & 1u) {
                     const int plane = (m >> 9) & 15;
 #pragma unroll
                     for (int q = 0; q < 6; q++) c.blocks[(size_t)(plane * 6 + q) * N + i] = C[q];
                 }
-                V3 dvj = s.warm ? ld3(c.dv, N, j) : V3{0.0f, 0.0f, 0.0f};
-                V3 wd = {dvj.x - dvi.x, dvj.y - dvi.y, dvj.z - dvi.z};
+                V3 wd = {in.dv.x - dvi.x, in.dv.y - dvi.y, in.dv.z - dvi.z};
                 V3 a = symmv(C, wd);
                 adv = {adv.x + a.x, adv.y + a.y, adv.z + a.z};
                 dg = {dg.x - C[0], dg.y - C[1], dg.z - C[2]};
@@ -455,9 +478,10 @@ __global__ void k_reset_barrier(unsigned* bar) {
     if (threadIdx.x == 0) *bar = 0u;
 }
 
-// Iteration count and status of the solve (one 64-thread block).
-__global__ void k_cg_finish(DevCloth c, StepScalars s) {
-    __shared__ float bc;
+// Iteration count and status of the solve, by all threads of one block
+// (uniform control flow: partial_sum synchronises the block).  Bits are
+// OR-ed in, so update blocks may flag non-finite values concurrently.
+__device__ inline void cg_finish_block(const DevCloth& c, const StepScalars& s, float* bc) {
     const int done = c.ctl[0];
     int it;
     float delta;
@@ -466,24 +490,37 @@ __global__ void k_cg_finish(DevCloth c, StepScalars s) {
         delta = c.scal[it];
     } else {
         it = s.max_it;
-        delta = partial_sum(c.del_part + (size_t)it * c.nt, c.nt, &bc);
+        delta = partial_sum(c.del_part + (size_t)it * c.nt, c.nt, bc);
     }
-    // r0.r0: written by the first spmv launch, or equal to delta without iterations
+    // r0.r0: written by the first spmv phase, or equal to delta without iterations
     const float delta0 = s.max_it > 0 ? c.scal[0] : delta;
     if (threadIdx.x == 0) {
         const float thresh = s.tol_mode == 1 ? s.tol2 * delta0 : s.tol2;
-        int st = c.ctl[2];
+        const int st = c.ctl[2];
+        int extra = 0;
         if (s.tol_mode != 2 && delta > thresh && !(st & OC_SOLVE_BREAKDOWN))
-            st |= OC_SOLVE_NOT_CONVERGED;
-        if (!isfinite(delta)) st |= OC_SOLVE_NONFINITE;
+            extra |= OC_SOLVE_NOT_CONVERGED;
+        if (!isfinite(delta)) extra |= OC_SOLVE_NONFINITE;
         c.ctl[1] = it;
-        c.ctl[2] = st;
-        c.ctl[3] |= st;
+        atomicOr(c.ctl + 2, extra);
+        atomicOr(c.ctl + 3, st | extra);
     }
 }
 
-// v += dv, x += h v, optional ground clamp (IE:693-699).
+// Iteration count and status of the solve (one 64-thread block).
+__global__ void k_cg_finish(DevCloth c, StepScalars s) {
+    __shared__ float bc;
+    cg_finish_block(c, s, &bc);
+}
+
+// v += dv, x += h v, optional ground clamp (IE:693-699).  FIN: block 0 also
+// records the solve's iteration count and status (the step's last launch).
+template <bool FIN>
 __global__ __launch_bounds__(256) void k_update(DevCloth c, StepScalars s) {
+    if (FIN && blockIdx.x == 0) {
+        __shared__ float bc;
+        cg_finish_block(c, s, &bc);
+    }
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= c.N) return;
     const int N = c.N;
@@ -657,7 +694,7 @@ int persistent_cg_grid(const DevCloth& c) {
     return std::min(grid, c.nt);
 }
 
-hipError_t launch_cg(const DevCloth& c, const StepScalars& s, hipStream_t st) {
+hipError_t launch_cg(const DevCloth& c, const StepScalars& s, hipStream_t st, bool finish) {
     hipError_t e;
     if (c.persistent && s.max_it > 0) {
         if (c.persistent > c.nt) return hipErrorInvalidValue;  // a block without a tile
@@ -671,17 +708,20 @@ hipError_t launch_cg(const DevCloth& c, const StepScalars& s, hipStream_t st) {
             reinterpret_cast<const void*>(k_cg_persistent<kTileX, kTileY, 12>), dim3(c.persistent),
             dim3(kTileX * kTileY), args, 0, st);
         if (e != hipSuccess) return e;
-        return launch_cg_finish(c, s, st);
+        return finish ? launch_cg_finish(c, s, st) : hipSuccess;
     }
     for (int k = 0; k < s.max_it; k++) {
         if ((e = launch_cg_spmv(c, s, k, st)) != hipSuccess) return e;
         if ((e = launch_cg_axpy(c, s, k, st)) != hipSuccess) return e;
     }
-    return launch_cg_finish(c, s, st);
+    return finish ? launch_cg_finish(c, s, st) : hipSuccess;
 }
 
-hipError_t launch_update(const DevCloth& c, const StepScalars& s, hipStream_t st) {
-    hipLaunchKernelGGL(k_update, dim3((c.N + 255) / 256), dim3(256), 0, st, c, s);
+hipError_t launch_update(const DevCloth& c, const StepScalars& s, hipStream_t st, bool finish) {
+    if (finish)
+        hipLaunchKernelGGL(k_update<true>, dim3((c.N + 255) / 256), dim3(256), 0, st, c, s);
+    else
+        hipLaunchKernelGGL(k_update<false>, dim3((c.N + 255) / 256), dim3(256), 0, st, c, s);
     return hipGetLastError();
 }
 

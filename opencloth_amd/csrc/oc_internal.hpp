@@ -93,11 +93,14 @@ struct StepScalars {
 // Kernel launchers (implicit.hip).  All return hipError_t of the launch.
 hipError_t launch_forces(const DevCloth& c, const StepScalars& s, hipStream_t st);
 hipError_t launch_assemble(const DevCloth& c, const StepScalars& s, hipStream_t st);
-hipError_t launch_cg(const DevCloth& c, const StepScalars& s, hipStream_t st);
+// finish = false leaves the iteration count and status to launch_update(...,
+// finish = true), the step's last launch.
+hipError_t launch_cg(const DevCloth& c, const StepScalars& s, hipStream_t st, bool finish = true);
 hipError_t launch_cg_spmv(const DevCloth& c, const StepScalars& s, int k, hipStream_t st);
 hipError_t launch_cg_axpy(const DevCloth& c, const StepScalars& s, int k, hipStream_t st);
 hipError_t launch_cg_finish(const DevCloth& c, const StepScalars& s, hipStream_t st);
-hipError_t launch_update(const DevCloth& c, const StepScalars& s, hipStream_t st);
+hipError_t launch_update(const DevCloth& c, const StepScalars& s, hipStream_t st,
+                         bool finish = false);
 // Buffers of one adjoint step (SoA vectors of the cloth's size).
 struct AdjointBufs {
     const float* gx_out;  // dL/dx after the step
