@@ -12,7 +12,10 @@ state resident in HBM.  Prints one JSON line:
                stream the kernels run on
   roofline     for the dominant kernel (cg_spmv): algorithmic bytes per
                launch (88N + 32P, the spmv share of the per-iteration model
-               160N + 32P) / its average launch time, against 8 TB/s
+               160N + 32P) / its average launch time (300 back-to-back
+               launches between two HIP events on the kernels' stream),
+               against 8 TB/s; traffic = HBM bytes per launch from the
+               committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
   cpu_baseline the FP32 CPU restatement of the same step (oracle, one core)
                on a bounded sample of the same workload
 """
@@ -58,17 +61,12 @@ def read_traffic(path: str):
 
 
 def copy_probe(mib: int) -> int:
-    """One torch device-to-device copy of mib MiB (an elementwise copy kernel
-    that reads and writes mib MiB once each); returns the bytes copied."""
-    import torch
+    """One device-to-device copy of mib MiB (hipMemcpy, run by the HIP
+    runtime as a copy kernel that reads and writes mib MiB once each);
+    returns the bytes copied."""
+    from opencloth_amd.hiptime import device_copy
 
-    n = mib * (1 << 20) // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    b.copy_(a)
-    torch.cuda.synchronize()
-    return n * 4
+    return device_copy(mib << 20)
 
 
 def cpu_baseline(sc, budget_s: float = 12.0) -> dict:
@@ -248,7 +246,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="after the timing, run one plain device copy of MIB MiB (a kernel of "
+                    help="after the timing, run one device-to-device copy of MIB MiB (a kernel of "
                          "known HBM traffic, the scale for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -276,8 +274,10 @@ def main():
     ms_per_step = ms / args.steps
 
     prof = cl.profile_steps(min(args.steps, 20))
-    spmv_ms, spmv_n = prof["cg_spmv"]
-    spmv_avg_s = spmv_ms / max(spmv_n, 1) * 1e-3
+    # the dominant kernel's launch time: back-to-back launches between two
+    # events (an event after every launch, as profile_steps does for the
+    # per-kernel breakdown, adds its own gap to each launch)
+    spmv_avg_s = cl.time_cg_spmv(300) * 1e-6
     achieved = spmv_bytes(N, P) / spmv_avg_s / 1e9
     traffic = read_traffic(os.path.join(ROOT, "profiles", "cg_spmv_512x512_pmc.txt"))
 
@@ -314,6 +314,7 @@ def main():
             "traffic": traffic,
             "algorithmic_bytes_per_launch": spmv_bytes(N, P),
             "avg_launch_us": spmv_avg_s * 1e6,
+            "traffic_GBps": None if traffic is None else traffic / spmv_avg_s / 1e9,
         },
     }
     if not args.no_batch:

@@ -156,6 +156,14 @@ int oc_cloth_synchronize(oc_cloth cloth);
 /* Status of the last solve: CG iterations and OC_SOLVE_* bits; synchronises. */
 int oc_cloth_last_status(oc_cloth cloth, int32_t* iters, int32_t* status);
 
+/* Measurement (no counterpart in the reference).  Average duration of the
+ * CG SpMV launch: assemble at the current state, run one CG iteration, then
+ * time `reps` back-to-back launches of the second iteration's SpMV (same
+ * inputs each time) between two HIP events on the handle's stream.  x and v
+ * are unchanged; dv and the CG vectors are left as that iteration leaves
+ * them.  Needs cg_max_iters >= 2; synchronises. */
+int oc_cloth_time_cg_spmv(oc_cloth cloth, int32_t reps, float* avg_us);
+
 /* ---- post-step passes of StepPhysics (IE:774-779) ----------------------- */
 /* Ellipsoid collider: transform and inverse as column-major 4x4 matrices
  * (glm::mat4, IE:541-544), object-space center and radius (IE:85-86). */

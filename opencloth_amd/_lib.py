@@ -48,6 +48,7 @@ SIGNATURES = {
     "oc_cloth_synchronize": (C.c_int, [_P]),
     "oc_cloth_last_status": (C.c_int, [_P, _P, _P]),
     "oc_cloth_profile_steps": (C.c_int, [_P, _I, _P, _P]),
+    "oc_cloth_time_cg_spmv": (C.c_int, [_P, _I, _P]),
     "oc_cloth_set_spring_stiffness": (C.c_int, [_P, _P]),
     "oc_cloth_get_spring_stiffness": (C.c_int, [_P, _P]),
     "oc_cloth_rollout_forward": (C.c_int, [_P, _I, _I]),

@@ -155,6 +155,13 @@ class ImplicitCloth:
         check(lib().oc_cloth_profile_steps(self.h, int(nsteps), ms, n))
         return {k: (ms[i], n[i]) for i, k in enumerate(self.KERNEL_CLASSES)}
 
+    def time_cg_spmv(self, reps: int = 200) -> float:
+        """Average microseconds of one CG SpMV launch over reps back-to-back
+        launches (oc_cloth_time_cg_spmv)."""
+        t = C.c_float()
+        check(lib().oc_cloth_time_cg_spmv(self.h, int(reps), C.byref(t)))
+        return t.value
+
     def last_status(self):
         it = C.c_int32(); st = C.c_int32()
         check(lib().oc_cloth_last_status(self.h, C.byref(it), C.byref(st)))

@@ -450,6 +450,34 @@ int oc_cloth_profile_steps(oc_cloth c, int32_t nsteps, float* ms, int32_t* launc
     return rc;
 }
 
+int oc_cloth_time_cg_spmv(oc_cloth c, int32_t reps, float* avg_us) {
+    if (!c || !avg_us) return fail(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (reps < 1) return fail(OC_ERR_INVALID_ARGUMENT, "reps < 1");
+    const oc::StepScalars s = scalars(c, c->prm.dt);
+    if (s.max_it < 2) return fail(OC_ERR_INVALID_ARGUMENT, "cg_max_iters must be >= 2");
+    OC_HIP(oc::launch_assemble(c->dc, s, c->stream));
+    OC_HIP(oc::launch_cg_spmv(c->dc, s, 0, c->stream));
+    OC_HIP(oc::launch_cg_axpy(c->dc, s, 0, c->stream));
+    OC_HIP(oc::launch_cg_spmv(c->dc, s, 1, c->stream));  // warm-up
+    hipEvent_t e0, e1;
+    OC_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(OC_ERR_DEVICE, "hipEventCreate failed");
+    }
+    bool ok = hipEventRecord(e0, c->stream) == hipSuccess;
+    for (int r = 0; ok && r < reps; r++) ok = oc::launch_cg_spmv(c->dc, s, 1, c->stream) == hipSuccess;
+    float ms = 0.0f;
+    ok = ok && hipEventRecord(e1, c->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+         hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->assembled = false;
+    if (!ok) return fail(OC_ERR_DEVICE, "timed SpMV launches failed");
+    *avg_us = 1000.0f * ms / (float)reps;
+    return OC_OK;
+}
+
 int oc_cloth_ellipsoid_collision(oc_cloth c, const oc_ellipsoid* e) {
     if (!c || !e) return fail(OC_ERR_INVALID_ARGUMENT, "NULL argument");
     OC_HIP(oc::launch_ellipsoid(c->dc.x, c->dc.v, c->dc.N, *e, 0, c->stream));

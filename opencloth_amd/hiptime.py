@@ -25,6 +25,11 @@ def hip() -> C.CDLL:
         _hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
         _hip.hipEventDestroy.argtypes = [C.c_void_p]
         _hip.hipStreamSynchronize.argtypes = [C.c_void_p]
+        _hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        _hip.hipFree.argtypes = [C.c_void_p]
+        _hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipDeviceSynchronize.argtypes = []
     return _hip
 
 
@@ -59,3 +64,27 @@ class Event:
 
 def device_synchronize():
     hip().hipDeviceSynchronize()
+
+
+def device_copy(nbytes: int) -> int:
+    """One device-to-device hipMemcpy of nbytes between two fresh buffers
+    (the runtime runs it as a copy kernel that reads and writes nbytes once;
+    the source is filled first by hipMemset).  Returns nbytes."""
+    h = hip()
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        rc = h.hipMalloc(C.byref(p), C.c_size_t(nbytes))
+        if rc != 0:
+            raise RuntimeError(f"hipMalloc failed ({rc})")
+    try:
+        for call, args in (("hipMemset", (a, C.c_int(1), C.c_size_t(nbytes))),
+                           ("hipDeviceSynchronize", ()),
+                           ("hipMemcpy", (b, a, C.c_size_t(nbytes), C.c_int(3))),  # DeviceToDevice
+                           ("hipDeviceSynchronize", ())):
+            rc = getattr(h, call)(*args)
+            if rc != 0:
+                raise RuntimeError(f"{call} failed ({rc})")
+    finally:
+        h.hipFree(a)
+        h.hipFree(b)
+    return nbytes
