@@ -211,6 +211,16 @@ int oc_cloth_create(int32_t nx, int32_t ny, const float* x0, const float* v0,
     d.wp = t.wp;
     d.persistent = (params->flags & OC_FLAG_CG_PERSISTENT) ? oc::persistent_cg_grid(d) : 0;
 
+    // the CG and assembly kernels address every array with 32-bit byte offsets
+    const uint64_t largest = std::max<uint64_t>(
+        {4ull * 6 * N * (uint64_t)std::max(t.n_planes, 1), 4ull * N * (uint64_t)std::max(t.wa, 1),
+         2ull * N * (uint64_t)std::max(t.wp, 1), 4ull * S, 12ull * N});
+    if (largest >= (1ull << 32)) {
+        delete c;
+        return fail(OC_ERR_UNSUPPORTED_TOPOLOGY,
+                    "cloth too large: a device array would exceed 4 GiB (" + std::to_string(largest) +
+                        " bytes)");
+    }
     const size_t vec = align_up(sizeof(float) * 3 * N);
     const size_t blocks = align_up(sizeof(float) * 6 * N * std::max(t.n_planes, 1));
     const size_t ainc_s = align_up(sizeof(int32_t) * std::max<size_t>(t.ainc_spring.size(), 1));

@@ -86,59 +86,98 @@ __global__ __launch_bounds__(TX* TY) void k_forces(DevCloth c, StepScalars s) {
 // ---------------------------------------------------------------------------
 // assembly + CG start
 // ---------------------------------------------------------------------------
-// Neighbour data of one incident spring: the other end's x, v (and dv for a
-// warm start) and the spring's constants.
-struct SpringIn {
-    V3 x, v, dv;
+// Constants of one incident spring (gathered by spring id).
+struct SpringC {
     float L0, ks, kd;
 };
 
 template <int WA>
-__device__ inline SpringIn fetch_spring(const DevCloth& c, const StepScalars& s, const int (&sps)[WA],
-                                        const uint16_t (&ms)[WA], int i, int k) {
-    SpringIn r{};
+__device__ inline SpringC fetch_constants(const DevCloth& c, const int (&sps)[WA], int k) {
+    SpringC r{0.0f, 0.0f, 0.0f};
     const int sp = sps[k];
     if (sp >= 0) {
-        const int N = c.N;
-        const int j = i + slot_dj(ms[k]) * c.nx + slot_di(ms[k]);
-        r.x = ld3(c.x, N, j);
-        r.v = ld3(c.v, N, j);
-        r.dv = s.warm ? ld3(c.dv, N, j) : V3{0.0f, 0.0f, 0.0f};
-        r.L0 = c.L0[sp];
-        r.ks = c.ks[sp];
-        r.kd = c.kd[sp];
+        r.L0 = ldo(c.L0, (uint32_t)sp);
+        r.ks = ldo(c.ks, (uint32_t)sp);
+        r.kd = ldo(c.kd, (uint32_t)sp);
     }
     return r;
 }
 
+// Tile + halo of x, v and (warm start) dv in LDS.
+template <int TX, int TY>
+struct AssembleShared {
+    static constexpr int WX = TX + 2 * H, WY = TY + 2 * H;
+    float wx[3][WY][WX];
+    float wv[3][WY][WX];
+    float wdv[3][WY][WX];
+    float ws[TX * TY / 64];
+};
+
 template <int TX, int TY, int WA>
 __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) {
     constexpr int NT = TX * TY;
-    __shared__ float ws[NT / 64];
+    constexpr int WX = TX + 2 * H, WY = TY + 2 * H;
+    constexpr int NWIN = (WX * WY + NT - 1) / NT;
+    __shared__ AssembleShared<TX, TY> sh;
     const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
     const int tile = tile_of_block(blockIdx.x, c.nt);
     const int bx = tile % c.ntx, by = tile / c.ntx;
     const int gi = bx * TX + lx, gj = by * TY + ly;
+    const bool own = gi < c.nx && gj < c.ny;
+    const int N = c.N, i = gj * c.nx + gi;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         c.ctl[0] = 0;
         c.ctl[1] = 0;
         c.ctl[2] = 0;
     }
-    float t = 0.0f;
-    if (gi < c.nx && gj < c.ny) {
-        const int N = c.N, i = gj * c.nx + gi;
-        const float h = s.h, hh = s.hh;
-        int sps[WA];
-        uint16_t ms[WA];
+    // own slot tables and the first springs' constants, then the window
+    int sps[WA];
+    uint16_t ms[WA];
 #pragma unroll
-        for (int k = 0; k < WA; k++) {
-            sps[k] = k < c.wa ? c.ainc_s[k * N + i] : -1;
-            ms[k] = k < c.wa ? c.ainc_m[k * N + i] : kEmptySlot;
+    for (int k = 0; k < WA; k++) {
+        sps[k] = (own && k < c.wa) ? ldo(c.ainc_s, (uint32_t)(k * N + i)) : -1;
+        ms[k] = (own && k < c.wa) ? ldo(c.ainc_m, (uint32_t)(k * N + i)) : kEmptySlot;
+    }
+    constexpr int kPrefetch = 4;
+    SpringC pre[kPrefetch];
+#pragma unroll
+    for (int k = 0; k < kPrefetch; k++) pre[k] = fetch_constants(c, sps, k);
+    const int x0 = bx * TX - H, y0 = by * TY - H;
+#pragma unroll
+    for (int n = 0; n < NWIN; n++) {
+        const int idx = threadIdx.x + n * NT;
+        if (idx < WX * WY) {
+            const int wx = idx % WX, wy = idx / WX;
+            const int gx = x0 + wx, gy = y0 + wy;
+            V3 xw = {0.0f, 0.0f, 0.0f}, vw = xw, dw = xw;
+            if (gx >= 0 && gx < c.nx && gy >= 0 && gy < c.ny) {
+                const int g = gy * c.nx + gx;
+                xw = ld3o(c.x, N, g);
+                vw = ld3o(c.v, N, g);
+                if (s.warm) dw = ld3o(c.dv, N, g);
+            }
+            sh.wx[0][wy][wx] = xw.x;
+            sh.wx[1][wy][wx] = xw.y;
+            sh.wx[2][wy][wx] = xw.z;
+            sh.wv[0][wy][wx] = vw.x;
+            sh.wv[1][wy][wx] = vw.y;
+            sh.wv[2][wy][wx] = vw.z;
+            sh.wdv[0][wy][wx] = dw.x;
+            sh.wdv[1][wy][wx] = dw.y;
+            sh.wdv[2][wy][wx] = dw.z;
         }
-        const bool pinned = c.pin[i] != 0;
-        V3 xi = ld3(c.x, N, i), vi = ld3(c.v, N, i);
-        V3 dvi = s.warm ? ld3(c.dv, N, i) : V3{0.0f, 0.0f, 0.0f};
-        if (!s.warm) st3(c.dv, N, i, dvi);
+    }
+    __syncthreads();
+
+    float t = 0.0f;
+    if (own) {
+        const float h = s.h, hh = s.hh;
+        const bool pinned = ldo(c.pin, (uint32_t)i) != 0;
+        const int cy = ly + H, cx = lx + H;
+        const V3 xi = {sh.wx[0][cy][cx], sh.wx[1][cy][cx], sh.wx[2][cy][cx]};
+        const V3 vi = {sh.wv[0][cy][cx], sh.wv[1][cy][cx], sh.wv[2][cy][cx]};
+        const V3 dvi = {sh.wdv[0][cy][cx], sh.wdv[1][cy][cx], sh.wdv[2][cy][cx]};
+        if (!s.warm) st3o(c.dv, N, i, dvi);
         V3 acc = {0.0f, 0.0f, 0.0f}, kv = {0.0f, 0.0f, 0.0f};
         if (!pinned) acc = {0.0f + s.mass * s.gx, 0.0f + s.mass * s.gy, 0.0f + s.mass * s.gz};
         V3 adv = {s.mass * dvi.x, s.mass * dvi.y, s.mass * dvi.z};
@@ -147,36 +186,37 @@ __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) 
         // Every spring is evaluated from this vertex's side (x1 = x_i): when
         // i is the spring's p2 that negates dp, f and K(v_i - v_j) exactly
         // and leaves K and P unchanged, so adding them equals subtracting
-        // the p1-side values bit for bit.  The neighbour data of the next
-        // kPrefetch springs is loaded ahead of the one being evaluated.
-        constexpr int kPrefetch = 2;
-        SpringIn pre[kPrefetch];
-#pragma unroll
-        for (int k = 0; k < kPrefetch; k++) pre[k] = fetch_spring(c, s, sps, ms, i, k);
+        // the p1-side values bit for bit.  Neighbours come from the LDS
+        // window; spring constants are loaded kPrefetch springs ahead.
 #pragma unroll
         for (int k = 0; k < WA; k++) {
             const int sp = sps[k];
             if (sp < 0) break;
             const uint16_t m = ms[k];
-            const SpringIn in = pre[0];
+            const SpringC in = pre[0];
 #pragma unroll
             for (int q = 0; q + 1 < kPrefetch; q++) pre[q] = pre[q + 1];
-            if (k + kPrefetch < WA) pre[kPrefetch - 1] = fetch_spring(c, s, sps, ms, i, k + kPrefetch);
+            if (k + kPrefetch < WA) pre[kPrefetch - 1] = fetch_constants(c, sps, k + kPrefetch);
+            const int wy = cy + slot_dj(m), wx = cx + slot_di(m);
+            const V3 xj = {sh.wx[0][wy][wx], sh.wx[1][wy][wx], sh.wx[2][wy][wx]};
+            const V3 vj = {sh.wv[0][wy][wx], sh.wv[1][wy][wx], sh.wv[2][wy][wx]};
             V3 f;
             float K[6], P[6];
-            spring_terms(xi, in.x, vi, in.v, in.L0, in.ks, in.kd, f, K, P);
+            spring_terms(xi, xj, vi, vj, in.L0, in.ks, in.kd, f, K, P);
 #pragma unroll
             for (int q = 0; q < 6; q++) C[q] += h * (in.kd * P[q]) + hh * K[q];
-            const V3 kw = symmv(K, V3{vi.x - in.v.x, vi.y - in.v.y, vi.z - in.v.z});
+            const V3 kw = symmv(K, V3{vi.x - vj.x, vi.y - vj.y, vi.z - vj.z});
             acc = {acc.x + f.x, acc.y + f.y, acc.z + f.z};
             kv = {kv.x + kw.x, kv.y + kw.y, kv.z + kw.z};
             if ((m >> 13) & 1u) {  // last spring of this pair
                 if ((m >> 14) & 1u) {
                     const int plane = (m >> 9) & 15;
 #pragma unroll
-                    for (int q = 0; q < 6; q++) c.blocks[(size_t)(plane * 6 + q) * N + i] = C[q];
+                    for (int q = 0; q < 6; q++)
+                        sto(c.blocks + (size_t)q * N, (uint32_t)(plane * 6 * N + i), C[q]);
                 }
-                V3 wd = {in.dv.x - dvi.x, in.dv.y - dvi.y, in.dv.z - dvi.z};
+                const V3 dvj = {sh.wdv[0][wy][wx], sh.wdv[1][wy][wx], sh.wdv[2][wy][wx]};
+                V3 wd = {dvj.x - dvi.x, dvj.y - dvi.y, dvj.z - dvi.z};
                 V3 a = symmv(C, wd);
                 adv = {adv.x + a.x, adv.y + a.y, adv.z + a.z};
                 dg = {dg.x - C[0], dg.y - C[1], dg.z - C[2]};
@@ -188,22 +228,22 @@ __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) 
         b.x = pinned ? 0.0f : h * (acc.x + h * kv.x);
         b.y = pinned ? 0.0f : h * (acc.y + h * kv.y);
         b.z = pinned ? 0.0f : h * (acc.z + h * kv.z);
-        st3(c.b, N, i, b);
+        st3o(c.b, N, i, b);
         V3 r = pinned ? V3{0.0f, 0.0f, 0.0f} : V3{b.x - adv.x, b.y - adv.y, b.z - adv.z};
-        st3(c.r, N, i, r);
+        st3o(c.r, N, i, r);
         if (s.flags & OC_FLAG_JACOBI) {
             const V3 pv = {jacobi_inv(dg.x), jacobi_inv(dg.y), jacobi_inv(dg.z)};
-            st3(c.pinv, N, i, pv);
+            st3o(c.pinv, N, i, pv);
             const V3 z = {pv.x * r.x, pv.y * r.y, pv.z * r.z};
-            st3(c.s, N, i, z);
-            st3(c.d0, N, i, z);
+            st3o(c.s, N, i, z);
+            st3o(c.d0, N, i, z);
             t = dot3(r, z);
         } else {
-            st3(c.d0, N, i, r);
+            st3o(c.d0, N, i, r);
             t = dot3(r, r);
         }
     }
-    float p = tile_sum<NT>(t, ws);
+    float p = tile_sum<NT>(t, sh.ws);
     if (threadIdx.x == 0) c.del_part[tile] = p;
 }
 
@@ -254,10 +294,10 @@ __device__ inline bool spmv_phase(const DevCloth& c, const StepScalars& s, int k
         if (idx < WX * WY && gx >= 0 && gx < c.nx && gy >= 0 && gy < c.ny) {
             const int g = gy * c.nx + gx;
             if (k == 0) {
-                wr[n] = ld3(c.d0, N, g);
+                wr[n] = ld3o(c.d0, N, g);
             } else {
-                wr[n] = ld3(zsrc, N, g);
-                wp[n] = ld3(dprev, N, g);
+                wr[n] = ld3o(zsrc, N, g);
+                wp[n] = ld3o(dprev, N, g);
             }
         }
     }
@@ -265,19 +305,19 @@ __device__ inline bool spmv_phase(const DevCloth& c, const StepScalars& s, int k
     float C[WP][6];
 #pragma unroll
     for (int sl = 0; sl < WP; sl++) {
-        m[sl] = (own && sl < c.wp) ? c.pair_m[sl * N + i] : kEmptySlot;
+        m[sl] = (own && sl < c.wp) ? ldo(c.pair_m, (uint32_t)(sl * N + i)) : kEmptySlot;
         if (m[sl] != kEmptySlot) {
             const int j = i + slot_dj(m[sl]) * c.nx + slot_di(m[sl]);
             const int blk = ((m[sl] >> 12) & 1) ? i : j;
-            const float* bp = c.blocks + (size_t)((m[sl] >> 8) & 15) * 6 * N + blk;
+            const uint32_t b0 = (uint32_t)((m[sl] >> 8) & 15) * 6u * (uint32_t)N + (uint32_t)blk;
 #pragma unroll
-            for (int q = 0; q < 6; q++) C[sl][q] = bp[(size_t)q * N];
+            for (int q = 0; q < 6; q++) C[sl][q] = ldo(c.blocks + (size_t)q * N, b0);
         } else {
 #pragma unroll
             for (int q = 0; q < 6; q++) C[sl][q] = 0.0f;
         }
     }
-    const bool pinned = own ? c.pin[i] != 0 : true;
+    const bool pinned = own ? ldo(c.pin, (uint32_t)i) != 0 : true;
 
     // ---- r.r of this iteration, stopping test, beta
     if (threadIdx.x < 64) {
@@ -311,7 +351,7 @@ __device__ inline bool spmv_phase(const DevCloth& c, const StepScalars& s, int k
                 d = {wr[n].x + beta * wp[n].x, wr[n].y + beta * wp[n].y, wr[n].z + beta * wp[n].z};
                 const int gx = x0 + wx, gy = y0 + wy;
                 if (wx >= H && wx < H + TX && wy >= H && wy < H + TY && gx < c.nx && gy < c.ny)
-                    st3(dcur, N, gy * c.nx + gx, d);
+                    st3o(dcur, N, gy * c.nx + gx, d);
             }
             sh.wd[0][wy][wx] = d.x;
             sh.wd[1][wy][wx] = d.y;
@@ -334,7 +374,7 @@ __device__ inline bool spmv_phase(const DevCloth& c, const StepScalars& s, int k
             acc = {acc.x + a.x, acc.y + a.y, acc.z + a.z};
         }
         V3 q = pinned ? V3{0.0f, 0.0f, 0.0f} : acc;
-        st3(c.q, N, i, q);
+        st3o(c.q, N, i, q);
         t = dot3(di, q);
     }
     float p = tile_sum<NT>(t, sh.ws);
@@ -371,11 +411,11 @@ __device__ inline bool axpy_phase(const DevCloth& c, const StepScalars& s, int k
     const bool jac = (s.flags & OC_FLAG_JACOBI) != 0;
     V3 d = {0.0f, 0.0f, 0.0f}, q = d, x = d, r = d, jv = d;
     if (own) {
-        d = ld3(dcur, N, i);
-        q = ld3(c.q, N, i);
-        x = ld3(c.dv, N, i);
-        r = ld3(c.r, N, i);
-        if (jac) jv = ld3(c.pinv, N, i);
+        d = ld3o(dcur, N, i);
+        q = ld3o(c.q, N, i);
+        x = ld3o(c.dv, N, i);
+        r = ld3o(c.r, N, i);
+        if (jac) jv = ld3o(c.pinv, N, i);
     }
     if (threadIdx.x < 64) {
         const float sum = partial_finish(c.sig_part + (size_t)k * nt, nt, pv);
@@ -398,11 +438,11 @@ __device__ inline bool axpy_phase(const DevCloth& c, const StepScalars& s, int k
     if (own) {
         x = {x.x + alpha * d.x, x.y + alpha * d.y, x.z + alpha * d.z};
         r = {r.x - alpha * q.x, r.y - alpha * q.y, r.z - alpha * q.z};
-        st3(c.dv, N, i, x);
-        st3(c.r, N, i, r);
+        st3o(c.dv, N, i, x);
+        st3o(c.r, N, i, r);
         if (jac) {
             const V3 z = {jv.x * r.x, jv.y * r.y, jv.z * r.z};
-            st3(c.s, N, i, z);
+            st3o(c.s, N, i, z);
             t = dot3(r, z);
         } else {
             t = dot3(r, r);
@@ -524,12 +564,12 @@ __global__ __launch_bounds__(256) void k_update(DevCloth c, StepScalars s) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= c.N) return;
     const int N = c.N;
-    V3 v = ld3(c.v, N, i), dv = ld3(c.dv, N, i), x = ld3(c.x, N, i);
+    V3 v = ld3o(c.v, N, i), dv = ld3o(c.dv, N, i), x = ld3o(c.x, N, i);
     v = {v.x + dv.x, v.y + dv.y, v.z + dv.z};
     x = {x.x + s.h * v.x, x.y + s.h * v.y, x.z + s.h * v.z};
     if ((s.flags & OC_FLAG_GROUND_CLAMP) && x.y < 0.0f) x.y = 0.0f;
-    st3(c.v, N, i, v);
-    st3(c.x, N, i, x);
+    st3o(c.v, N, i, v);
+    st3o(c.x, N, i, x);
     if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) && isfinite(v.y) &&
           isfinite(v.z))) {
         atomicOr(c.ctl + 2, OC_SOLVE_NONFINITE);

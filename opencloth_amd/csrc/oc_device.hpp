@@ -21,6 +21,29 @@ __device__ inline void st3(float* p, int N, int i, V3 v) {
     p[N + i] = v.y;
     p[2 * N + i] = v.z;
 }
+
+// Element idx of a device array, addressed by a 32-bit byte offset from the
+// array's base.  With a uniform base this is the scalar-base form of a global
+// load (no 64-bit address arithmetic per element).  Callers guarantee that
+// the array is below 4 GiB (checked when a cloth is created).
+template <typename T>
+__device__ inline T ldo(const T* p, uint32_t idx) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + idx * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ inline void sto(T* p, uint32_t idx, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + idx * (uint32_t)sizeof(T)) = v;
+}
+// SoA vec3 with the component planes as uniform bases: one 32-bit offset
+// serves all three components.
+__device__ inline V3 ld3o(const float* p, int N, int i) {
+    return {ldo(p, (uint32_t)i), ldo(p + N, (uint32_t)i), ldo(p + 2 * (size_t)N, (uint32_t)i)};
+}
+__device__ inline void st3o(float* p, int N, int i, V3 v) {
+    sto(p, (uint32_t)i, v.x);
+    sto(p + N, (uint32_t)i, v.y);
+    sto(p + 2 * (size_t)N, (uint32_t)i, v.z);
+}
 __device__ inline V3 add3(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 __device__ inline V3 sub3(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 __device__ inline V3 scl3(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
