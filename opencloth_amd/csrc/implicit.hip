@@ -461,17 +461,19 @@ __global__ __launch_bounds__(TX* TY) void k_cg_axpy(DevCloth c, StepScalars s, i
 }
 
 // Grid barrier of the persistent solve: every block adds 1 to *bar and waits
-// until it reaches target.  The wait is bounded; on time-out the block sets
-// ctl[4] and the caller ends the kernel, so every wave exits.
+// until it reaches target.  The add is a release (the block's writes reach
+// the device-coherent level first); the wait polls with relaxed agent-scope
+// loads, which read the coherent value without invalidating caches, and a
+// single acquire fence follows.  The wait is bounded; on time-out the block
+// sets ctl[4] and the caller ends the kernel, so every wave exits.
 __device__ inline bool grid_barrier(unsigned* bar, unsigned target, int* err) {
     __syncthreads();
     __shared__ int ok;
     if (threadIdx.x == 0) {
-        __threadfence();
         __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         int good = 1;
         unsigned spins = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             __builtin_amdgcn_s_sleep(1);
             if (++spins > (1u << 24)) {
                 good = 0;
@@ -479,7 +481,7 @@ __device__ inline bool grid_barrier(unsigned* bar, unsigned target, int* err) {
                 break;
             }
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         ok = good;
     }
     __syncthreads();
