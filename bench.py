@@ -69,6 +69,75 @@ def copy_probe(mib: int) -> int:
     return device_copy(mib << 20)
 
 
+def bench_entries(steps: int = 100) -> dict:
+    """The other entry points on 512x512 cloths (HIP events around `steps`
+    steps after 5 warm-up steps), with simple algorithmic byte models:
+      verlet  the Verlet demo's constants on the 512x512 grid (perturbed,
+              top edge pinned, damping -0.0125, ground clamp), one fused
+              forces + integration launch per step; bytes 36N + 12S (x and
+              x_last read, next positions written, spring constants once)
+      pbd     the PBD demo's constraints and constants on a 512x512 grid
+              (mass 1/N, 2 solver iterations), one launch per colour; bytes
+              ns(72 D + 108 B) + 132N (distance: two particles and inverse
+              masses read, indices, rest length and k' read, two particles
+              written; bending likewise with three; prediction, damping sums
+              and integration 132N)
+      implicit_step_physics  the implicit demo's full StepPhysics (21x21,
+              step + collider + Provot), launch-bound."""
+    from opencloth_amd import scenes
+    from opencloth_amd.cloth import ImplicitCloth, PBDCloth, VerletCloth
+    from opencloth_amd.hiptime import Event
+
+    def timed(run, n):
+        run(5)
+        e0, e1 = Event(), Event()
+        e0.record()
+        run(n)
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_ms(e1) / n
+
+    out = {}
+    n = 512
+    X0 = scenes.grid_positions(n, n, 4.0, 5.0)
+    springs = scenes.grid_springs(n, n, X0, ks=(50.75, 50.75, 50.95), kd=(-0.25, -0.25, -0.25))
+    X = scenes.perturb(X0, 1e-3 * 4.0 / (n - 1), 0)
+    pinned = np.zeros(n * n, np.uint8)
+    pinned[:n] = 1
+    vc = VerletCloth(n, n, X, springs, pinned, mass=1.0, gravity=(0.0, -0.00981, 0.0),
+                     damping=-0.0125, dt=1.0 / 60.0, ground_clamp=True)
+    ms = timed(lambda k: vc.step(k), steps)
+    N, S = n * n, len(springs)
+    b = 36 * N + 12 * S
+    out["verlet"] = {"workload": f"{n}x{n} Verlet step, {S} springs", "ms_per_step": ms,
+                     "vertex_steps_per_s": N / (ms * 1e-3), "algorithmic_bytes": b,
+                     "roofline_frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    vc.close()
+
+    sc = scenes.pbd_large_scene(n)
+    pc = PBDCloth.from_scene(sc)
+    nd_col, nb_col = pc.num_colors()
+    ms = timed(lambda k: pc.step(k), steps)
+    ns = sc.params.solver_iterations
+    b = ns * (72 * len(sc.distance) + 108 * len(sc.bending)) + 132 * N
+    out["pbd"] = {"workload": f"{n}x{n} PBD step, {len(sc.distance)} distance and {len(sc.bending)} "
+                              f"bending constraints, {ns} solver iterations",
+                  "ms_per_step": ms, "vertex_steps_per_s": N / (ms * 1e-3),
+                  "colours": [nd_col, nb_col],
+                  "launches_per_step": 5 + ns * (nd_col + nb_col + int(sc.params.ground)) + 1,
+                  "algorithmic_bytes": b, "roofline_frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    pc.close()
+
+    sc = scenes.demo_scene()
+    cl = ImplicitCloth.from_scene(sc)
+    ell, slack = sc.extra["ellipsoid"], sc.extra["provot_slack"]
+    ms = timed(lambda k: cl.step_physics(k, ell, slack), steps)
+    out["implicit_step_physics"] = {"workload": "21x21 implicit demo StepPhysics (step, collider, Provot)",
+                                    "ms_per_step": ms}
+    cl.close()
+    return out
+
+
 def cpu_baseline(sc, budget_s: float = 12.0) -> dict:
     """FP32 CPU restatement (one core) timed on whole steps of the same
     workload until about budget_s seconds have passed."""
@@ -244,6 +313,8 @@ def main():
     ap.add_argument("--workload", default="large", choices=["large", "plain"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
+    ap.add_argument("--no-entries", action="store_true",
+                    help="skip the Verlet / PBD / StepPhysics timings")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the timing, run one device-to-device copy of MIB MiB (a kernel of "
@@ -320,6 +391,8 @@ def main():
     if not args.no_batch:
         out["batch"] = bench_batch(args.batch_steps, 2)
         out["batch_adjoint"] = bench_batch_adjoint()
+    if not args.no_entries:
+        out["entries"] = bench_entries()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc)
     if args.copy_probe > 0:

@@ -37,12 +37,6 @@ namespace {
 constexpr int H = kHalo;
 using namespace dev;
 
-// Blocks are dispatched round-robin over the 8 XCDs; each XCD gets a
-// contiguous band of tiles so that halo and shared-block re-reads hit its L2.
-__device__ inline int tile_of_block(int b, int nt) {
-    if ((nt & 7) == 0) return (b & 7) * (nt >> 3) + (b >> 3);
-    return b;
-}
 
 // Jacobi preconditioner entry 1/diag(A); a non-positive or non-finite
 // diagonal (A can be indefinite under strong compression) falls back to 1.

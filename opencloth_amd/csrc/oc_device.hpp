@@ -11,6 +11,13 @@ namespace dev {
 
 constexpr int kPartialRegs = 16;  // tile partials per lane loaded up front
 
+// Blocks are dispatched round-robin over the 8 XCDs; each XCD gets a
+// contiguous band of tiles so that halo and shared-data re-reads hit its L2.
+__device__ inline int tile_of_block(int b, int nt) {
+    if ((nt & 7) == 0) return (b & 7) * (nt >> 3) + (b >> 3);
+    return b;
+}
+
 struct V3 {
     float x, y, z;
 };

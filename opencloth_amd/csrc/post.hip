@@ -6,8 +6,10 @@
 //                 reference only reads X there, so the gather equals its
 //                 sequential scatter bit for bit)
 //   k_verlet    : ComputeForces(dt) + IntegrateVerlet (VL:428-484) fused per
-//                 vertex; reads x, x_last and writes the next positions into
-//                 a third buffer, so no thread sees a half-updated neighbour
+//                 vertex on 32x8 tiles whose positions and velocities (with a
+//                 2-wide halo) are staged in LDS; reads x, x_last and writes
+//                 the next positions into a third buffer, so no thread sees a
+//                 half-updated neighbour
 //
 // Expressions follow the reference's GLM order, built with -ffp-contract=off.
 #include "oc_device.hpp"
@@ -97,48 +99,105 @@ __global__ __launch_bounds__(256) void k_provot(DevCloth c, float slack) {
     if (touched) st3(c.v, N, i, v);
 }
 
-// Force on vertex i with Verlet velocities (VL:448-484), gather in list order.
-__device__ inline V3 verlet_force(const DevVerlet& c, const VerletScalars& s, int i) {
-    const int N = c.N;
-    const V3 xi = ld3(c.x, N, i), xli = ld3(c.xl, N, i);
-    const V3 vi = {(xi.x - xli.x) / s.dt, (xi.y - xli.y) / s.dt, (xi.z - xli.z) / s.dt};
-    V3 F = {0.0f, 0.0f, 0.0f};
+// Verlet step on a 2D tile with a kHalo-wide halo staged in LDS: positions
+// and Verlet velocities v = (x - x_last)/dt of the window are computed once
+// (the same division per vertex as evaluating them per spring end).
+template <int TX, int TY>
+struct VerletShared {
+    static constexpr int WX = TX + 2 * kHalo, WY = TY + 2 * kHalo;
+    float wx[3][WY][WX];
+    float wv[3][WY][WX];
+};
+
+// Force on the tile's vertex (VL:448-484): gravity on free vertices, damping
+// on all, springs in list order (pinned vertices: damping only).  Springs are
+// evaluated from this vertex's side (x1 = x_i), which is bitwise the negated
+// p1-side force, so it is always added.  FORCES: write the force; otherwise
+// integrate (VL:428-444) into the next positions.
+template <int TX, int TY, int WA, bool FORCES>
+__global__ __launch_bounds__(TX* TY) void k_verlet(DevVerlet c, VerletScalars s) {
+    constexpr int NT = TX * TY, H = kHalo;
+    constexpr int WX = TX + 2 * H, WY = TY + 2 * H;
+    constexpr int NWIN = (WX * WY + NT - 1) / NT;
+    __shared__ VerletShared<TX, TY> sh;
+    const int ntx = (c.nx + TX - 1) / TX, nt = ntx * ((c.ny + TY - 1) / TY);
+    const int tile = tile_of_block(blockIdx.x, nt);
+    const int bx = tile % ntx, by = tile / ntx;
+    const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+    const int gi = bx * TX + lx, gj = by * TY + ly;
+    const bool own = gi < c.nx && gj < c.ny;
+    const int N = c.N, i = gj * c.nx + gi;
+    int sps[WA];
+    uint16_t ms[WA];
+#pragma unroll
+    for (int k = 0; k < WA; k++) {
+        sps[k] = (own && k < c.wa) ? c.ainc_s[k * N + i] : -1;
+        ms[k] = (own && k < c.wa) ? c.ainc_m[k * N + i] : kEmptySlot;
+    }
+    const int x0 = bx * TX - H, y0 = by * TY - H;
+#pragma unroll
+    for (int n = 0; n < NWIN; n++) {
+        const int idx = threadIdx.x + n * NT;
+        if (idx < WX * WY) {
+            const int wx = idx % WX, wy = idx / WX;
+            const int gx = x0 + wx, gy = y0 + wy;
+            V3 xw = {0.0f, 0.0f, 0.0f}, vw = xw;
+            if (gx >= 0 && gx < c.nx && gy >= 0 && gy < c.ny) {
+                const int g = gy * c.nx + gx;
+                xw = ld3(c.x, N, g);
+                const V3 xl = ld3(c.xl, N, g);
+                vw = {(xw.x - xl.x) / s.dt, (xw.y - xl.y) / s.dt, (xw.z - xl.z) / s.dt};
+            }
+            sh.wx[0][wy][wx] = xw.x;
+            sh.wx[1][wy][wx] = xw.y;
+            sh.wx[2][wy][wx] = xw.z;
+            sh.wv[0][wy][wx] = vw.x;
+            sh.wv[1][wy][wx] = vw.y;
+            sh.wv[2][wy][wx] = vw.z;
+        }
+    }
+    __syncthreads();
+    if (!own) return;
+    const int cy = ly + H, cx = lx + H;
+    const V3 xi = {sh.wx[0][cy][cx], sh.wx[1][cy][cx], sh.wx[2][cy][cx]};
+    const V3 vi = {sh.wv[0][cy][cx], sh.wv[1][cy][cx], sh.wv[2][cy][cx]};
     const bool pinned = c.pin[i] != 0;
+    V3 F = {0.0f, 0.0f, 0.0f};
     if (!pinned) F = {F.x + s.gx * s.mass, F.y + s.gy * s.mass, F.z + s.gz * s.mass};
     F = {F.x + s.damping * vi.x, F.y + s.damping * vi.y, F.z + s.damping * vi.z};
-    if (pinned) return F;
-    for (int k = 0; k < c.wa; k++) {
-        const int sp = c.ainc_s[k * N + i];
-        if (sp < 0) break;
-        const uint16_t m = c.ainc_m[k * N + i];
-        const int j = i + slot_dj(m) * c.nx + slot_di(m);
-        const V3 xj = ld3(c.x, N, j), xlj = ld3(c.xl, N, j);
-        const V3 vj = {(xj.x - xlj.x) / s.dt, (xj.y - xlj.y) / s.dt, (xj.z - xlj.z) / s.dt};
-        const bool second = (m >> 8) & 1u;
-        const V3 f = second ? spring_force(xj, xi, vj, vi, c.L0[sp], c.ks[sp], c.kd[sp])
-                            : spring_force(xi, xj, vi, vj, c.L0[sp], c.ks[sp], c.kd[sp]);
-        F = second ? sub3(F, f) : add3(F, f);
+    if (!pinned) {
+#pragma unroll
+        for (int k = 0; k < WA; k++) {
+            const int sp = sps[k];
+            if (sp < 0) break;
+            const int wy = cy + slot_dj(ms[k]), wx = cx + slot_di(ms[k]);
+            const V3 xj = {sh.wx[0][wy][wx], sh.wx[1][wy][wx], sh.wx[2][wy][wx]};
+            const V3 vj = {sh.wv[0][wy][wx], sh.wv[1][wy][wx], sh.wv[2][wy][wx]};
+            F = add3(F, spring_force(xi, xj, vi, vj, c.L0[sp], c.ks[sp], c.kd[sp]));
+        }
     }
-    return F;
-}
-
-__global__ __launch_bounds__(256) void k_verlet_forces(DevVerlet c, VerletScalars s) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= c.N) return;
-    st3(c.f, c.N, i, verlet_force(c, s, i));
-}
-
-__global__ __launch_bounds__(256) void k_verlet(DevVerlet c, VerletScalars s) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= c.N) return;
-    const int N = c.N;
-    const V3 F = verlet_force(c, s, i);
+    if (FORCES) {
+        st3(c.f, N, i, F);
+        return;
+    }
     const float c2 = (s.dt * s.dt) / s.mass;
-    const V3 x = ld3(c.x, N, i), xl = ld3(c.xl, N, i);
-    V3 xn = {x.x + (x.x - xl.x) + c2 * F.x, x.y + (x.y - xl.y) + c2 * F.y,
-             x.z + (x.z - xl.z) + c2 * F.z};
+    const V3 xl = ld3(c.xl, N, i);
+    V3 xn = {xi.x + (xi.x - xl.x) + c2 * F.x, xi.y + (xi.y - xl.y) + c2 * F.y,
+             xi.z + (xi.z - xl.z) + c2 * F.z};
     if ((s.flags & OC_FLAG_GROUND_CLAMP) && xn.y < 0.0f) xn.y = 0.0f;
     st3(c.xn, N, i, xn);
+}
+
+template <bool FORCES>
+hipError_t launch_verlet(const DevVerlet& c, const VerletScalars& s, hipStream_t st) {
+    const int nt = ((c.nx + kTileX - 1) / kTileX) * ((c.ny + kTileY - 1) / kTileY);
+    if (c.wa <= 16)
+        hipLaunchKernelGGL((k_verlet<kTileX, kTileY, 16, FORCES>), dim3(nt), dim3(kTileX * kTileY), 0, st, c,
+                           s);
+    else
+        hipLaunchKernelGGL((k_verlet<kTileX, kTileY, kMaxAssemblyWidth, FORCES>), dim3(nt),
+                           dim3(kTileX * kTileY), 0, st, c, s);
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -155,13 +214,11 @@ hipError_t launch_provot(const DevCloth& c, float slack, hipStream_t st) {
 }
 
 hipError_t launch_verlet_forces(const DevVerlet& c, const VerletScalars& s, hipStream_t st) {
-    hipLaunchKernelGGL(k_verlet_forces, dim3((c.N + 255) / 256), dim3(256), 0, st, c, s);
-    return hipGetLastError();
+    return launch_verlet<true>(c, s, st);
 }
 
 hipError_t launch_verlet_step(const DevVerlet& c, const VerletScalars& s, hipStream_t st) {
-    hipLaunchKernelGGL(k_verlet, dim3((c.N + 255) / 256), dim3(256), 0, st, c, s);
-    return hipGetLastError();
+    return launch_verlet<false>(c, s, st);
 }
 
 }  // namespace oc
