@@ -94,6 +94,12 @@ const char* oc_last_error(void);
 /* Number of visible HIP devices (0 when none). */
 int oc_device_count(void);
 
+/* Tile shape (tile_x by tile_y vertices, one workgroup each) of the
+ * single-cloth kernels.  The CG dot products are reduced per tile and then
+ * across tiles in a fixed order, so this shape is what a CPU restatement
+ * needs to reproduce their bits.  Host only. */
+int oc_cloth_tile_shape(int32_t* tile_x, int32_t* tile_y);
+
 /* Render triangles of an nx*ny grid: 2*(nx-1)*(ny-1) triangles, 3 vertex
  * indices each, written to out (host memory, 6*(nx-1)*(ny-1) int32), in the
  * reference's order and winding (IE:473-488) but 32-bit (the reference's

@@ -30,6 +30,7 @@ SIGNATURES = {
     "oc_last_error": (C.c_char_p, []),
     "oc_device_count": (C.c_int, []),
     "oc_grid_triangles": (C.c_int, [_I, _I, _P]),
+    "oc_cloth_tile_shape": (C.c_int, [_P, _P]),
     "oc_cloth_create": (C.c_int, [_I, _I, _P, _P, _P, _I, _P, _P, _P]),
     "oc_cloth_destroy": (C.c_int, [_P]),
     "oc_cloth_set_stream": (C.c_int, [_P, _P]),
@@ -171,6 +172,13 @@ def lib() -> C.CDLL:
 def check(rc: int) -> None:
     if rc != OC_OK:
         raise OpenClothError(rc, lib().oc_last_error().decode(errors="replace"))
+
+
+def tile_shape():
+    """(tile_x, tile_y) of the single-cloth kernels (oc_cloth_tile_shape)."""
+    tx, ty = C.c_int32(), C.c_int32()
+    check(lib().oc_cloth_tile_shape(C.byref(tx), C.byref(ty)))
+    return tx.value, ty.value
 
 
 def ptr(a):

@@ -160,6 +160,13 @@ int oc_device_count(void) {
     return n;
 }
 
+int oc_cloth_tile_shape(int32_t* tile_x, int32_t* tile_y) {
+    if (!tile_x || !tile_y) return fail(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    *tile_x = oc::kTileX;
+    *tile_y = oc::kTileY;
+    return OC_OK;
+}
+
 int oc_grid_triangles(int32_t nx, int32_t ny, int32_t* out) {
     if (!out) return fail(OC_ERR_INVALID_ARGUMENT, "out is NULL");
     if (nx < 2 || ny < 2) return fail(OC_ERR_INVALID_ARGUMENT, "grid must be at least 2x2");

@@ -176,9 +176,18 @@ hipError_t launch_pbd_bending(const DevPbd& c, const PbdScalars& s, int first, i
 hipError_t launch_pbd_ground(const DevPbd& c, hipStream_t st);
 hipError_t launch_pbd_integrate(const DevPbd& c, const PbdScalars& s, hipStream_t st);
 
-// Tile shape of the implicit kernels.
-constexpr int kTileX = 32;
-constexpr int kTileY = 8;
+// Tile shape of the implicit kernels (one workgroup per tile; a multiple of
+// 64 threads).  The reduction order depends on it; oc_cloth_tile_shape
+// reports it.  64x8 measured fastest at 512x512 among 32x8, 32x16, 32x32,
+// 64x4, 64x8, 64x16, 128x4 and 128x8.
+#ifndef OC_TILE_X
+#define OC_TILE_X 64
+#endif
+#ifndef OC_TILE_Y
+#define OC_TILE_Y 8
+#endif
+constexpr int kTileX = OC_TILE_X;
+constexpr int kTileY = OC_TILE_Y;
 
 // ---- batched cloths --------------------------------------------------------
 // All cloths' vertices are concatenated (SoA planes of stride Ntot); slot

@@ -8,8 +8,15 @@ import numpy as np
 
 from opencloth_amd.scenes import StepParams
 
-# tile shape of the HIP implicit kernels (opencloth_amd/csrc/oc_internal.hpp)
-TILE_X, TILE_Y = 32, 8
+def _tile_shape():
+    """The single-cloth kernels' tile shape, as the library reports it (the
+    reduction order of the CG dots depends on it); host only, no GPU call."""
+    from opencloth_amd._lib import tile_shape
+
+    return tile_shape()
+
+
+TILE_X, TILE_Y = _tile_shape()
 
 
 def oracle_params(p: StepParams, nx: int, ny: int, gpu_order: bool = True):
