@@ -76,3 +76,12 @@ def test_pbd_create_rejects_bad_constraints_without_device():
     p.solver_iterations = 0
     rc = lib().oc_pbd_create(4, ptr(X), None, ptr(d), 1, ptr(b), 0, ctypes.byref(p), ctypes.byref(h))
     assert rc == OC_ERR_INVALID_ARGUMENT and not h.value
+
+
+def test_tile_shape_is_reported_without_device():
+    """oc_cloth_tile_shape (host only) gives the shape the oracle must use for
+    the kernels' reduction order: whole waves per tile."""
+    from opencloth_amd._lib import tile_shape
+
+    tx, ty = tile_shape()
+    assert tx > 0 and ty > 0 and (tx * ty) % 64 == 0 and tx * ty <= 1024
