@@ -194,6 +194,13 @@ def bench_batch(steps: int, warmup: int, n_cloths: int = 4096, plain_steps: int 
     it, st = b.status()
     b.close()
     N = int(sc.X.shape[0])
+    # algorithmic bytes of the last step: the single-cloth model per cloth with
+    # its own CG iteration count (P = S - (nx + ny): one duplicate bend per
+    # grid row and column)
+    nxs, nys, ss = sc.nx.astype(np.int64), sc.ny.astype(np.int64), sc.n_springs.astype(np.int64)
+    ns = nxs * nys
+    ps = ss - (nxs + nys)
+    step_b = int(np.sum(61 * ns + 12 * ss + 32 * ps + it.astype(np.int64) * (160 * ns + 32 * ps) + 60 * ns))
 
     plain = [ImplicitCloth.from_scene(sc.cloth(c)) for c in range(sc.n_cloths)]
     for cl in plain:
@@ -219,6 +226,8 @@ def bench_batch(steps: int, warmup: int, n_cloths: int = 4096, plain_steps: int 
         "schedule": {"block_threads": sched[0], "vertices_per_thread": sched[1]},
         "cg_iters_mean": float(it.mean()), "cg_iters_max": int(it.max()),
         "status_nonzero": int((st != 0).sum()),
+        "step_algorithmic_bytes": step_b,
+        "step_roofline_frac": step_b / (ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "plain_ms_per_step": ms_plain,
         "plain_vertex_steps_per_s": N / (ms_plain * 1e-3),
         "speedup_vs_plain": ms_plain / (ms / steps),
