@@ -16,8 +16,9 @@ state resident in HBM.  Prints one JSON line:
                launches between two HIP events on the kernels' stream),
                against 8 TB/s; traffic = HBM bytes per launch from the
                committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
-  cpu_baseline the FP32 CPU restatement of the same step (oracle, one core)
-               on a bounded sample of the same workload
+  cpu_baseline the FP32 CPU restatement of the same step (oracle) on all
+               host threads (OpenMP) and on one, each on a bounded sample of
+               the same workload
 """
 from __future__ import annotations
 
@@ -138,31 +139,56 @@ def bench_entries(steps: int = 100) -> dict:
     return out
 
 
-def cpu_baseline(sc, budget_s: float = 12.0) -> dict:
-    """FP32 CPU restatement (one core) timed on whole steps of the same
-    workload until about budget_s seconds have passed."""
+def cpu_threads() -> int:
+    """Host threads for the CPU baseline: the affinity mask, capped by
+    OMP_NUM_THREADS when that is set."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(env))) if env.isdigit() else n
+
+
+def cpu_baseline(sc, budget_s: float = 12.0, single_budget_s: float = 6.0) -> dict:
+    """FP32 CPU restatement timed on whole steps of the same workload: on all
+    host threads (OpenMP loops over vertices, springs and tiles, tile-tree
+    dot products) for about budget_s seconds, and on one thread with
+    sequential dot products, as the reference sums (IE:219-225), for about
+    single_budget_s seconds."""
     from oracle import Oracle, make_params
 
     o = Oracle(32)
     sysm = o.system(sc.n_vertices, sc.springs, sc.pinned)
-    prm = make_params(sc.params)
-    X = o.arr(sc.X).copy(); V = o.arr(sc.V).copy(); DV = np.zeros_like(X)
-    sysm.step(X, V, DV, prm)  # warm-up
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        sysm.step(X, V, DV, prm)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or n >= 50:
-            break
+
+    def run(threads, prm, budget):
+        o.set_threads(threads)
+        X = o.arr(sc.X).copy(); V = o.arr(sc.V).copy(); DV = np.zeros_like(X)
+        sysm.step(X, V, DV, prm)  # warm-up
+        n = 0
+        t0 = time.perf_counter()
+        while True:
+            sysm.step(X, V, DV, prm)
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= budget or n >= 100:
+                break
+        o.set_threads(1)
+        return sc.n_vertices * n / el, n, el
+
+    from opencloth_amd._lib import tile_shape
+
+    nth = cpu_threads()
+    tx, ty = tile_shape()
+    v_mt, n_mt, el_mt = run(nth, make_params(sc.params, nx=sc.nx, ny=sc.ny, tx=tx, ty=ty, dot_mode=1),
+                            budget_s)
+    v_st, n_st, el_st = run(1, make_params(sc.params), single_budget_s)
     return {
-        "value": sc.n_vertices * n / el,
+        "value": v_mt,
         "unit": "vertex-steps/s",
-        "cores": 1,
+        "cores": nth,
         "kind": "port",
-        "sample": f"{n} steps of the same {sc.nx}x{sc.ny} workload after 1 warm-up step "
-                  f"(FP32 restatement, sequential dots, single thread, {el:.1f} s)",
+        "sample": f"{n_mt} steps of the same {sc.nx}x{sc.ny} workload after 1 warm-up step on {nth} "
+                  f"threads (FP32 restatement, OpenMP, tile-tree dots, {el_mt:.1f} s); "
+                  f"single thread with sequential dots: {n_st} steps, {el_st:.1f} s",
+        "single_thread_value": v_st,
     }
 
 

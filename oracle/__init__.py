@@ -79,6 +79,7 @@ class Oracle:
         L.orc_cg_blockdiag.argtypes = [C.c_int, _P, _P, _P, C.c_int, C.c_double]
         L.orc_cg_reference_blockdiag.argtypes = [C.c_int, _P, _P, _P, C.c_int, C.c_double]
         L.orc_step.argtypes = [_P, _P, _P, _P, _P, _P]
+        L.orc_set_threads.argtypes = [C.c_int]
         L.orc_rollout.argtypes = [_P, _P, _P, _P, _P, C.c_int, _P, _P]
         L.orc_set_ks.argtypes = [_P, _P]
         L.orc_ellipsoid.argtypes = [_P, _P, _P, _P, _P, _P, C.c_float, C.c_int]
@@ -98,6 +99,11 @@ class Oracle:
         L.orc_pbd_project.argtypes = [_P, _P, _P, C.c_int]
         L.orc_pbd_integrate.argtypes = [_P, _P, _P, _P, _P]
         L.orc_pbd_step.argtypes = [_P, _P, _P, _P, C.c_int, C.c_int, _P, _P, _P, C.c_float]
+
+    def set_threads(self, n: int) -> int:
+        """Worker threads of the implicit step's parallel loops (results are
+        bitwise the same for any count); returns the previous count."""
+        return int(self.L.orc_set_threads(int(n)))
 
     # -- arrays --------------------------------------------------------------
     def arr(self, a) -> np.ndarray:

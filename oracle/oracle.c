@@ -52,6 +52,25 @@ typedef OC_REAL real;
 
 #define EXPORT __attribute__((visibility("default")))
 
+/* Worker threads of the per-vertex, per-spring and per-tile loops (OpenMP
+ * when built with -fopenmp; 1 by default).  Every parallel loop writes
+ * disjoint outputs and every sum keeps its serial order (per-pair sums in
+ * list order, dot-product tiles combined in tile order, mode 0 dots and the
+ * batched order serial), so results are bitwise independent of the count.
+ * Pinned by: tests/test_oracle_threads.py (1 vs 4 threads, bit for bit). */
+static int g_threads = 1;
+#ifdef _OPENMP
+#define OMP_FOR _Pragma("omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)")
+#else
+#define OMP_FOR
+#endif
+
+EXPORT int orc_set_threads(int n) {
+    int old = g_threads;
+    g_threads = n > 0 ? n : 1;
+    return old;
+}
+
 /* Spring record in the reference's own layout (IE:68-73, 24 bytes).
  * Kd follows the reference's sign convention: negative for damping
  * (IE:310-312), so a damping coefficient kd > 0 is stored as Kd = -kd. */
@@ -503,15 +522,22 @@ EXPORT void orc_assemble(const orc_sys* s, const real* X, const real* V, const o
                          real* B, real* Cp, real* diag) {
     real h = (real)prm->h, hh = h * h, m = (real)prm->mass;
     real gv[3] = {(real)prm->gx, (real)prm->gy, (real)prm->gz};
-    /* pair blocks, in list order within each pair */
+    /* pair blocks, in list order within each pair (each spring's block is
+     * evaluated first, then the blocks are added serially) */
+    real* Cs = (real*)malloc(sizeof(real) * 6 * (s->S > 0 ? s->S : 1));
+    OMP_FOR
+    for (int i = 0; i < s->S; i++) {
+        real f[3], K[6];
+        spring_block(s, &s->springs[i], X, V, h, hh, f, K, Cs + 6 * i);
+    }
     for (int p = 0; p < s->P; p++)
         for (int c = 0; c < 6; c++) Cp[6 * p + c] = 0;
     for (int i = 0; i < s->S; i++) {
-        real f[3], K[6], C[6];
-        spring_block(s, &s->springs[i], X, V, h, hh, f, K, C);
         int p = s->spring_pair[i];
-        for (int c = 0; c < 6; c++) Cp[6 * p + c] += C[c];
+        for (int c = 0; c < 6; c++) Cp[6 * p + c] += Cs[6 * i + c];
     }
+    free(Cs);
+    OMP_FOR
     for (int i = 0; i < s->N; i++) {
         real acc[3] = {0, 0, 0}, kv[3] = {0, 0, 0};
         if (!s->pinned[i])
@@ -552,6 +578,7 @@ EXPORT void orc_assemble(const orc_sys* s, const real* X, const real* V, const o
  * Pinned by: the dense-matrix tests (q equals the dense A times d). */
 EXPORT void orc_matvec(const orc_sys* s, const real* Cp, double mass, const real* d, real* q) {
     real m = (real)mass;
+    OMP_FOR
     for (int i = 0; i < s->N; i++) {
         real acc[3] = {m * d[3 * i + 0], m * d[3 * i + 1], m * d[3 * i + 2]};
         for (int k = s->vp_ptr[i]; k < s->vp_ptr[i + 1]; k++) {
@@ -612,12 +639,14 @@ static real dot_vec(const orc_sys* s, const orc_params* prm, const real* a, cons
     int ntx = (nx + tx - 1) / tx, nty = (ny + ty - 1) / ty;
     int nt = ntx * nty;
     int nthr = tx * ty, nw = nthr / 64;
-    real lanes[64];
     real cross[64];
     for (int l = 0; l < 64; l++) cross[l] = 0;
+    real* part = (real*)malloc(sizeof(real) * (nt > 0 ? nt : 1));
+    OMP_FOR
     for (int t = 0; t < nt; t++) {
         int bx = t % ntx, by = t / ntx;
-        real part = 0;
+        real lanes[64];
+        real pt = 0;
         for (int w = 0; w < nw; w++) {
             for (int l = 0; l < 64; l++) {
                 int tid = w * 64 + l;
@@ -625,10 +654,12 @@ static real dot_vec(const orc_sys* s, const orc_params* prm, const real* a, cons
                 lanes[l] = (i < nx && j < ny) ? dot3(a + 3 * (j * nx + i), b + 3 * (j * nx + i)) : (real)0;
             }
             real ws = wave_tree(lanes);
-            part = (w == 0) ? ws : part + ws;
+            pt = (w == 0) ? ws : pt + ws;
         }
-        cross[t % 64] += part;
+        part[t] = pt;
     }
+    for (int t = 0; t < nt; t++) cross[t % 64] += part[t];
+    free(part);
     return wave_tree(cross);
 }
 
@@ -672,6 +703,7 @@ static int cg_core(int N, const uint8_t* pinned, matvec_fn mv, dot_fn dot, const
     real* q = (real*)malloc(sizeof(real) * 3 * (N > 0 ? N : 1));
     real* z = pinv ? (real*)malloc(sizeof(real) * 3 * (N > 0 ? N : 1)) : r; /* P^-1 r */
     mv(ctx, dv, q);
+    OMP_FOR
     for (int i = 0; i < 3 * N; i++) {
         r[i] = (pinned && pinned[i / 3]) ? (real)0 : b[i] - q[i];
         if (pinv) z[i] = pinv[i] * r[i];
@@ -693,6 +725,7 @@ static int cg_core(int N, const uint8_t* pinned, matvec_fn mv, dot_fn dot, const
         /* sigma == 0 only once r (hence d) has underflowed: keep the iteration
          * count fixed without producing 0/0 */
         real alpha = sigma == 0 ? (real)0 : delta / sigma;
+        OMP_FOR
         for (int i = 0; i < 3 * N; i++) {
             dv[i] = dv[i] + alpha * d[i];
             r[i] = r[i] - alpha * q[i];
@@ -701,6 +734,7 @@ static int cg_core(int N, const uint8_t* pinned, matvec_fn mv, dot_fn dot, const
         real delta_old = delta;
         delta = dot(ctx, r, z);
         real beta = delta_old == 0 ? (real)0 : delta / delta_old;
+        OMP_FOR
         for (int i = 0; i < 3 * N; i++) d[i] = z[i] + beta * d[i];
         it++;
     }
@@ -739,6 +773,7 @@ static real* jacobi_inverse(const orc_sys* s, const real* Cp, const orc_params* 
     if (!(prm->flags & ORC_FLAG_JACOBI)) return NULL;
     real* pinv = (real*)malloc(sizeof(real) * 3 * (s->N > 0 ? s->N : 1));
     real m = (real)prm->mass;
+    OMP_FOR
     for (int i = 0; i < s->N; i++) {
         real dg[3] = {m, m, m};
         for (int k = s->vp_ptr[i]; k < s->vp_ptr[i + 1]; k++)
@@ -852,6 +887,10 @@ EXPORT int orc_step(const orc_sys* s, real* X, real* V, real* DV, const orc_para
     int st = 0;
     int it = orc_cg(s, Cp, prm, B, DV, &st, NULL);
     real h = (real)prm->h;
+    int bad = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1) reduction(| : bad)
+#endif
     for (int i = 0; i < N; i++) {
         for (int c = 0; c < 3; c++) {
             V[3 * i + c] = V[3 * i + c] + DV[3 * i + c];
@@ -859,8 +898,9 @@ EXPORT int orc_step(const orc_sys* s, real* X, real* V, real* DV, const orc_para
         }
         if ((prm->flags & 1) && X[3 * i + 1] < 0) X[3 * i + 1] = 0;
         for (int c = 0; c < 3; c++)
-            if (!ISFINITE_(X[3 * i + c]) || !ISFINITE_(V[3 * i + c])) st |= ORC_NONFINITE;
+            if (!ISFINITE_(X[3 * i + c]) || !ISFINITE_(V[3 * i + c])) bad = 1;
     }
+    if (bad) st |= ORC_NONFINITE;
     free(B);
     free(Cp);
     if (status) *status = st;
