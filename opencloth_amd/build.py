@@ -62,7 +62,39 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+HOST_SRCS = [os.path.join(HERE, "host", "ie_demo.cpp")]
+
+
+def host_program(name: str) -> str:
+    """Path of a built host program (lib/<name>), built first if absent."""
+    exe = os.path.join(LIBDIR, name)
+    if not os.path.exists(exe):
+        if not os.path.exists(LIB):
+            build()
+        else:
+            build_host()
+    return exe
+
+
+def build_host(verbose: bool = False) -> list:
+    """C++ programs over the C ABI (host code only, g++): each links
+    lib/libopencloth.so and finds it next to itself at run time."""
+    outs = []
+    for s in HOST_SRCS:
+        exe = os.path.join(LIBDIR, os.path.splitext(os.path.basename(s))[0])
+        cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-ffp-contract=off", "-Wall",
+               "-I", os.path.join(ROOT, "include"), s, "-o", exe + ".tmp", "-L", LIBDIR, "-lopencloth",
+               "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(exe + ".tmp", exe)
+        outs.append(exe)
+    return outs
 
 
 if __name__ == "__main__":

@@ -138,4 +138,4 @@ def test_torch_autograd_function():
     gks, _, gv0 = cl.rollout_backward(2 * (cl.X - sc.extra["target"]))
     assert torch.allclose(ks.grad, torch.from_numpy(gks), rtol=1e-6, atol=0)
     assert torch.allclose(v0.grad, torch.from_numpy(gv0), rtol=1e-6, atol=1e-12)
-    assert float(loss) > 0
+    assert float(loss.detach()) > 0

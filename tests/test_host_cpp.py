@@ -1,0 +1,65 @@
+"""The C++ host layer (include/opencloth.hpp) and the implicit demo built on
+it (opencloth_amd/host/ie_demo.cpp): the program's scene, made by its own
+C++ restatement of InitGL (IE:454-544), equals the Python demo scene bit for
+bit, so the GPU test of its StepPhysics loop runs the oracle on the same
+inputs (CPU only: --scene-only makes no GPU call)."""
+import struct
+import subprocess
+
+import numpy as np
+
+from opencloth_amd import scenes
+from opencloth_amd.build import host_program
+
+
+def read_scene(path):
+    b = open(path, "rb").read()
+    n, s = struct.unpack("<ii", b[:8])
+    off = 8
+    X = np.frombuffer(b, np.float32, 3 * n, off).reshape(n, 3); off += 12 * n
+    springs = np.frombuffer(b, scenes.SPRING_DTYPE, s, off); off += 24 * s
+    pinned = np.frombuffer(b, np.uint8, n, off); off += n
+    ell = np.frombuffer(b, np.float32, 36, off); off += 144
+    assert off == len(b)
+    return X, springs, pinned, ell
+
+
+def test_demo_scene_matches(tmp_path):
+    out = tmp_path / "scene.bin"
+    r = subprocess.run([host_program("ie_demo"), "--scene-only", "--out", str(out)],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    X, springs, pinned, ell = read_scene(out)
+    sc = scenes.demo_scene()
+    e = sc.extra["ellipsoid"]
+    assert np.array_equal(X.view(np.uint32), sc.X.view(np.uint32))
+    assert np.array_equal(springs.view(np.uint8), sc.springs.view(np.uint8))
+    assert np.array_equal(pinned, sc.pinned)
+    assert np.array_equal(ell[:16].view(np.uint32), e["M"].view(np.uint32))
+    assert np.array_equal(ell[16:32].view(np.uint32), e["Minv"].view(np.uint32))
+    assert np.array_equal(ell[32:35], e["center"]) and ell[35] == e["radius"]
+
+
+def test_demo_rejects_bad_arguments(tmp_path):
+    r = subprocess.run([host_program("ie_demo"), "--steps"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+
+
+def test_header_compiles_with_a_glm_like_type(tmp_path):
+    """ImplicitEuler<Vec3> accepts any 12-byte three-float type and rejects
+    others at compile time (host compile only)."""
+    import os
+
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    good = tmp_path / "good.cpp"
+    good.write_text('#include "opencloth.hpp"\nstruct v3 { float x, y, z; };\n'
+                    "float* f(std::vector<v3>& a) { return opencloth::floats(a); }\n")
+    bad = tmp_path / "bad.cpp"
+    bad.write_text('#include "opencloth.hpp"\nstruct v4 { float x, y, z, w; };\n'
+                   "float* f(std::vector<v4>& a) { return opencloth::floats(a); }\n")
+    ok = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", inc, str(good)],
+                        capture_output=True, text=True)
+    assert ok.returncode == 0, ok.stderr
+    no = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", inc, str(bad)],
+                        capture_output=True, text=True)
+    assert no.returncode != 0 and "three packed floats" in no.stderr
