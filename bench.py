@@ -180,6 +180,19 @@ def cpu_baseline(sc, budget_s: float = 12.0, single_budget_s: float = 6.0) -> di
     v_mt, n_mt, el_mt = run(nth, make_params(sc.params, nx=sc.nx, ny=sc.ny, tx=tx, ty=ty, dot_mode=1),
                             budget_s)
     v_st, n_st, el_st = run(1, make_params(sc.params), single_budget_s)
+
+    # the reference's own IntegrateImplicit, bug for bug (spring-indexed
+    # block-diagonal A, IE:684-700), on the same inputs, one thread
+    o.set_threads(1)
+    X = o.arr(sc.X).copy(); V = o.arr(sc.V).copy(); Vn = np.zeros_like(X)
+    p = sc.params
+    n_lit, t0 = 0, time.perf_counter()
+    while True:
+        sysm.ie_literal_step(X, V, Vn, p.mass, p.gravity, p.dt, i_max=10, eps=1e-3)
+        n_lit += 1
+        el_lit = time.perf_counter() - t0
+        if el_lit >= 2.0 or n_lit >= 100:
+            break
     return {
         "value": v_mt,
         "unit": "vertex-steps/s",
@@ -189,6 +202,15 @@ def cpu_baseline(sc, budget_s: float = 12.0, single_budget_s: float = 6.0) -> di
                   f"threads (FP32 restatement, OpenMP, tile-tree dots, {el_mt:.1f} s); "
                   f"single thread with sequential dots: {n_st} steps, {el_st:.1f} s",
         "single_thread_value": v_st,
+        "reference_literal": {
+            "value": sc.n_vertices * n_lit / el_lit,
+            "unit": "vertex-steps/s",
+            "cores": 1,
+            "sample": f"{n_lit} steps of the reference's own ComputeForces + IntegrateImplicit "
+                      f"(IE:645-700) restated bug for bug (spring-indexed block-diagonal A, CG 10 "
+                      f"iterations, EPS 1e-3) on the same {sc.nx}x{sc.ny} inputs, {el_lit:.1f} s; "
+                      f"a different (incorrect) linear system, for the record",
+        },
     }
 
 

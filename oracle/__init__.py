@@ -80,6 +80,7 @@ class Oracle:
         L.orc_cg_reference_blockdiag.argtypes = [C.c_int, _P, _P, _P, C.c_int, C.c_double]
         L.orc_step.argtypes = [_P, _P, _P, _P, _P, _P]
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_ie_literal_step.argtypes = [_P, _P, _P, _P, C.c_double, _P, C.c_double, C.c_int, C.c_double]
         L.orc_rollout.argtypes = [_P, _P, _P, _P, _P, C.c_int, _P, _P]
         L.orc_set_ks.argtypes = [_P, _P]
         L.orc_ellipsoid.argtypes = [_P, _P, _P, _P, _P, _P, C.c_float, C.c_int]
@@ -245,6 +246,17 @@ class System:
                              _ptr(hist) if hist is not None else None, _ptr(iters))
         return X, V, DV, hist, iters, st
 
+
+    def ie_literal_step(self, X, V, Vnew, mass, gravity, h, i_max=10, eps=1e-3):
+        """The reference's own ComputeForces + IntegrateImplicit, bug for bug
+        (spring-indexed blocks), in place on X, V and the warm start Vnew;
+        returns the CG iterations.  For the record and the CPU timing."""
+        g = np.ascontiguousarray(gravity, np.float64)
+        it = self.o.L.orc_ie_literal_step(self.h, _ptr(X), _ptr(V), _ptr(Vnew), float(mass), _ptr(g),
+                                          float(h), int(i_max), float(eps))
+        if it < 0:
+            raise ValueError("the literal step needs at least as many springs as vertices")
+        return it
 
     def set_ks(self, ks):
         """Per-spring stiffness in the oracle's precision."""

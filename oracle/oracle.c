@@ -869,6 +869,67 @@ EXPORT int orc_cg_reference_blockdiag(int N, const real* A, const real* b, real*
 }
 
 /* ------------------------------------------------------------------------ */
+/* The reference's own step, bug for bug                                     */
+/* ------------------------------------------------------------------------ */
+/* ComputeForces + IntegrateImplicit of IE as written (IE:645-700), for the
+ * record and the CPU timing only; nothing in the library follows it:
+ *   F   = gravity on free vertices plus the spring forces, scattered in list
+ *         order with pinned ends skipped (IE:645-682);
+ *   A_i = M_i - h^2 K_i for i < N, K_i the Jacobian of SPRING i (IE:671):
+ *         LargeVector's operator- runs over M's N entries (IE:187-193,
+ *         IE:688), so spring i's block lands on vertex i;
+ *   b   = M V + h F (IE:689), with GLM's mat3*vec3 and vec3*float;
+ *   SolveConjugateGradient(A, V_new, b) on that block-diagonal system, warm
+ *         from the previous V_new (IE:227-249), then x += h V_new, clamp
+ *         y < 0 to 0, V = V_new (IE:693-699).
+ * The reference adds `gravity` as a force (IE:653); here g is an
+ * acceleration and the force is mass*g, which is the same float when mass
+ * is a power of two (the demo's 0.5).  Pins come from the pinned mask (the
+ * reference tests i != 0 && i != numX, IE:652, IE:677-680).  Needs S >= N.
+ * Returns the CG iterations, or -1 when S < N.
+ * Pinned by: tests/test_oracle_literal.py -- free fall with inert springs
+ * (closed form), and a three-spring system solved densely in float64 with
+ * each vertex's block built from its spring's Jacobian in the spectral form
+ * of IE:671 (-Ks along the spring, -Ks(1-L0/l) across). */
+EXPORT int orc_ie_literal_step(const orc_sys* s, real* X, real* V, real* Vnew, double mass,
+                               const double* g, double h, int i_max, double eps) {
+    int N = s->N;
+    if (s->S < N) return -1;
+    real m = (real)mass, dt = (real)h, dt2 = dt * dt;
+    real* F = (real*)malloc(sizeof(real) * 3 * (N > 0 ? N : 1));
+    real* A = (real*)malloc(sizeof(real) * 9 * (N > 0 ? N : 1));
+    real* b = (real*)malloc(sizeof(real) * 3 * (N > 0 ? N : 1));
+    orc_forces_scatter(s, X, V, mass, g, F);
+    for (int i = 0; i < N; i++) {
+        const orc_spring* sp = &s->springs[i];
+        real f[3], K[6];
+        spring_terms(X + 3 * sp->p1, X + 3 * sp->p2, V + 3 * sp->p1, V + 3 * sp->p2,
+                     (real)sp->rest_length, s->ks[i], (real)sp->Kd, f, K, NULL);
+        /* column-major 3x3 (glm::mat3), entry (row r, column c) at c*3+r */
+        const real Kf[9] = {K[0], K[3], K[4], K[3], K[1], K[5], K[4], K[5], K[2]};
+        real Mi[9];
+        for (int e = 0; e < 9; e++) {
+            Mi[e] = (e == 0 || e == 4 || e == 8) ? m : (real)0;
+            A[9 * i + e] = Mi[e] - Kf[e] * dt2;
+        }
+        for (int r = 0; r < 3; r++) {
+            real mv = Mi[r] * V[3 * i + 0] + Mi[3 + r] * V[3 * i + 1] + Mi[6 + r] * V[3 * i + 2];
+            b[3 * i + r] = mv + F[3 * i + r] * dt;
+        }
+    }
+    int it = orc_cg_reference_blockdiag(N, A, b, Vnew, i_max, eps);
+    for (int i = 0; i < N; i++) {
+        for (int c = 0; c < 3; c++) X[3 * i + c] = X[3 * i + c] + Vnew[3 * i + c] * dt;
+        if (X[3 * i + 1] < 0) X[3 * i + 1] = 0;
+        for (int c = 0; c < 3; c++) V[3 * i + c] = Vnew[3 * i + c];
+    }
+    free(F);
+    free(A);
+    free(b);
+    return it;
+}
+
+/* ------------------------------------------------------------------------ */
 /* One implicit step and rollouts                                            */
 /* ------------------------------------------------------------------------ */
 /* The north-star step: assemble, filtered CG for dv (warm start from DV when
