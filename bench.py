@@ -168,7 +168,7 @@ def cpu_baseline(sc, budget_s: float = 12.0, single_budget_s: float = 6.0) -> di
             sysm.step(X, V, DV, prm)
             n += 1
             el = time.perf_counter() - t0
-            if el >= budget or n >= 100:
+            if el >= budget or n >= 400:
                 break
         o.set_threads(1)
         return sc.n_vertices * n / el, n, el
