@@ -378,8 +378,9 @@ def main():
                          "known HBM traffic, the scale for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
-              "(see DESIGN.md for how the step would shard)", file=sys.stderr)
+        print("bench.py: measures one GPU only; a single cloth would shard into row strips with "
+              "2-row halo exchanges and two scalar all-reduces per CG iteration, a batch by "
+              "instance, and neither is built here", file=sys.stderr)
         sys.exit(2)
 
     from opencloth_amd import scenes

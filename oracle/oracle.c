@@ -932,7 +932,7 @@ EXPORT int orc_ie_literal_step(const orc_sys* s, real* X, real* V, real* Vnew, d
 /* ------------------------------------------------------------------------ */
 /* One implicit step and rollouts                                            */
 /* ------------------------------------------------------------------------ */
-/* The north-star step: assemble, filtered CG for dv (warm start from DV when
+/* The implicit step: assemble, filtered CG for dv (warm start from DV when
  * prm->warm_start, else from zero), then v += dv, x += h*v (IE:693-699 with
  * the block-CSR dv form of BW:781-792, velocity updated once by dv).
  * Optional ground clamp (flags bit0, IE:695-697).  DV receives the solution.
