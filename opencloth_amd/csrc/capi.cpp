@@ -233,9 +233,10 @@ int oc_cloth_create(int32_t nx, int32_t ny, const float* x0, const float* v0,
     const size_t ainc_s = align_up(sizeof(int32_t) * std::max<size_t>(t.ainc_spring.size(), 1));
     const size_t ainc_m = align_up(sizeof(uint16_t) * std::max<size_t>(t.ainc_meta.size(), 1));
     const size_t pair_m = align_up(sizeof(uint16_t) * std::max<size_t>(t.pair_meta.size(), 1));
+    const size_t pair_k = align_up(sizeof(uint16_t) * std::max<size_t>(t.pair_mask.size(), 1));
     const size_t sp = align_up(sizeof(float) * S);
     const size_t pin = align_up(N);
-    const size_t total = 11 * vec + blocks + ainc_s + ainc_m + pair_m + 3 * sp + pin;
+    const size_t total = 11 * vec + blocks + ainc_s + ainc_m + pair_m + pair_k + 3 * sp + pin;
     if (hipMalloc(&c->arena, total) != hipSuccess) {
         delete c;
         return fail(OC_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(total) + " bytes failed");
@@ -261,6 +262,9 @@ int oc_cloth_create(int32_t nx, int32_t ny, const float* x0, const float* v0,
     d.ainc_s = (const int32_t*)take(ainc_s);
     d.ainc_m = (const uint16_t*)take(ainc_m);
     d.pair_m = (const uint16_t*)take(pair_m);
+    d.pair_mask = (const uint16_t*)take(pair_k);
+    d.stencil = t.stencil ? 1 : 0;
+    for (int k = 0; k < oc::kStencil; k++) d.stencil_plane[k] = t.stencil_plane[k];
     d.L0 = (const float*)take(sp);
     d.ks = (const float*)take(sp);
     d.kd = (const float*)take(sp);
@@ -291,6 +295,9 @@ int oc_cloth_create(int32_t nx, int32_t ny, const float* x0, const float* v0,
                         hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy((void*)d.pair_m, t.pair_meta.data(), t.pair_meta.size() * 2,
                         hipMemcpyHostToDevice) == hipSuccess &&
+              (t.pair_mask.empty() ||
+               hipMemcpy((void*)d.pair_mask, t.pair_mask.data(), t.pair_mask.size() * 2,
+                         hipMemcpyHostToDevice) == hipSuccess) &&
               hipMemcpy((void*)d.L0, L0.data(), S * 4, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy((void*)d.ks, ks.data(), S * 4, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy((void*)d.kd, kd.data(), S * 4, hipMemcpyHostToDevice) == hipSuccess &&

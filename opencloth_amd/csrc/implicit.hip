@@ -254,8 +254,11 @@ struct SpmvShared {
 
 // One block's share of CG iteration k, part 1.  read_done: take the stop
 // flag of earlier launches from c.ctl[0] (multi-kernel form).  Returns true
-// when the solve has stopped (every block decides alike).
-template <int TX, int TY, int WP>
+// when the solve has stopped (every block decides alike).  STENCIL: the
+// cloth's pairs follow the kStencil order (c.stencil), so a 12-bit mask per
+// vertex replaces the WP slot metas and every slot's offset, owner and plane
+// are constants; the sums run in the same pair order either way.
+template <int TX, int TY, int WP, bool STENCIL = false>
 __device__ inline bool spmv_phase(const DevCloth& c, const StepScalars& s, int k, bool read_done,
                                   SpmvShared<TX, TY>& sh, int vb) {
     constexpr int NT = TX * TY;
@@ -269,7 +272,10 @@ __device__ inline bool spmv_phase(const DevCloth& c, const StepScalars& s, int k
     const bool own = gi < c.nx && gj < c.ny;
     const int i = gj * c.nx + gi;
 
-    // ---- loads that do not depend on beta
+    // ---- loads that do not depend on beta (the stencil mask first: the
+    // block loads wait for it)
+    uint32_t mask = 0;
+    if constexpr (STENCIL) mask = own ? ldo(c.pair_mask, (uint32_t)i) : 0u;
     const int done = read_done ? c.ctl[0] : 0;
     const float delta0 = c.scal[0];
     const float delta_old = k > 0 ? c.scal[k - 1] : 0.0f;
@@ -295,20 +301,37 @@ __device__ inline bool spmv_phase(const DevCloth& c, const StepScalars& s, int k
             }
         }
     }
-    uint16_t m[WP];
-    float C[WP][6];
+    constexpr int NS = STENCIL ? kStencil : WP;
+    uint16_t m[NS];
+    float C[NS][6];
+    if constexpr (STENCIL) {
 #pragma unroll
-    for (int sl = 0; sl < WP; sl++) {
-        m[sl] = (own && sl < c.wp) ? ldo(c.pair_m, (uint32_t)(sl * N + i)) : kEmptySlot;
-        if (m[sl] != kEmptySlot) {
-            const int j = i + slot_dj(m[sl]) * c.nx + slot_di(m[sl]);
-            const int blk = ((m[sl] >> 12) & 1) ? i : j;
-            const uint32_t b0 = (uint32_t)((m[sl] >> 8) & 15) * 6u * (uint32_t)N + (uint32_t)blk;
+        for (int k = 0; k < kStencil; k++) {
+            const int di = stencil_di(k), dj = stencil_dj(k);
+            const uint32_t blk = stencil_owner(k) ? (uint32_t)i : (uint32_t)(i + dj * c.nx + di);
+            const uint32_t off = blk + (uint32_t)c.stencil_plane[k] * 6u * (uint32_t)N;
+            if ((mask >> k) & 1u) {
 #pragma unroll
-            for (int q = 0; q < 6; q++) C[sl][q] = ldo(c.blocks + (size_t)q * N, b0);
-        } else {
+                for (int q = 0; q < 6; q++) C[k][q] = ldo(c.blocks, off + (uint32_t)(q * N));
+            } else {
 #pragma unroll
-            for (int q = 0; q < 6; q++) C[sl][q] = 0.0f;
+                for (int q = 0; q < 6; q++) C[k][q] = 0.0f;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int sl = 0; sl < WP; sl++) {
+            m[sl] = (own && sl < c.wp) ? ldo(c.pair_m, (uint32_t)(sl * N + i)) : kEmptySlot;
+            if (m[sl] != kEmptySlot) {
+                const int j = i + slot_dj(m[sl]) * c.nx + slot_di(m[sl]);
+                const int blk = ((m[sl] >> 12) & 1) ? i : j;
+                const uint32_t b0 = (uint32_t)((m[sl] >> 8) & 15) * 6u * (uint32_t)N + (uint32_t)blk;
+#pragma unroll
+                for (int q = 0; q < 6; q++) C[sl][q] = ldo(c.blocks + (size_t)q * N, b0);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 6; q++) C[sl][q] = 0.0f;
+            }
         }
     }
     const bool pinned = own ? ldo(c.pin, (uint32_t)i) != 0 : true;
@@ -359,13 +382,24 @@ __device__ inline bool spmv_phase(const DevCloth& c, const StepScalars& s, int k
     if (own) {
         const V3 di = {sh.wd[0][ly + H][lx + H], sh.wd[1][ly + H][lx + H], sh.wd[2][ly + H][lx + H]};
         V3 acc = {s.mass * di.x, s.mass * di.y, s.mass * di.z};
+        if constexpr (STENCIL) {
 #pragma unroll
-        for (int sl = 0; sl < WP; sl++) {
-            if (m[sl] == kEmptySlot) break;
-            const int wy = ly + H + slot_dj(m[sl]), wx = lx + H + slot_di(m[sl]);
-            V3 w = {sh.wd[0][wy][wx] - di.x, sh.wd[1][wy][wx] - di.y, sh.wd[2][wy][wx] - di.z};
-            V3 a = symmv(C[sl], w);
-            acc = {acc.x + a.x, acc.y + a.y, acc.z + a.z};
+            for (int k = 0; k < kStencil; k++) {
+                if (!((mask >> k) & 1u)) continue;
+                const int wy = ly + H + stencil_dj(k), wx = lx + H + stencil_di(k);
+                V3 w = {sh.wd[0][wy][wx] - di.x, sh.wd[1][wy][wx] - di.y, sh.wd[2][wy][wx] - di.z};
+                V3 a = symmv(C[k], w);
+                acc = {acc.x + a.x, acc.y + a.y, acc.z + a.z};
+            }
+        } else {
+#pragma unroll
+            for (int sl = 0; sl < WP; sl++) {
+                if (m[sl] == kEmptySlot) break;
+                const int wy = ly + H + slot_dj(m[sl]), wx = lx + H + slot_di(m[sl]);
+                V3 w = {sh.wd[0][wy][wx] - di.x, sh.wd[1][wy][wx] - di.y, sh.wd[2][wy][wx] - di.z};
+                V3 a = symmv(C[sl], w);
+                acc = {acc.x + a.x, acc.y + a.y, acc.z + a.z};
+            }
         }
         V3 q = pinned ? V3{0.0f, 0.0f, 0.0f} : acc;
         st3o(c.q, N, i, q);
@@ -376,10 +410,10 @@ __device__ inline bool spmv_phase(const DevCloth& c, const StepScalars& s, int k
     return false;
 }
 
-template <int TX, int TY, int WP>
+template <int TX, int TY, int WP, bool STENCIL>
 __global__ __launch_bounds__(TX* TY) void k_cg_spmv(DevCloth c, StepScalars s, int k) {
     __shared__ SpmvShared<TX, TY> sh;
-    spmv_phase<TX, TY, WP>(c, s, k, true, sh, blockIdx.x);
+    spmv_phase<TX, TY, WP, STENCIL>(c, s, k, true, sh, blockIdx.x);
 }
 
 // CG iteration k, part 2: dv += alpha d, r -= alpha q, partial r.r.  Returns
@@ -687,11 +721,14 @@ hipError_t launch_assemble(const DevCloth& c, const StepScalars& s, hipStream_t 
 }
 
 hipError_t launch_cg_spmv(const DevCloth& c, const StepScalars& s, int k, hipStream_t st) {
-    if (c.wp <= 12)
-        hipLaunchKernelGGL((k_cg_spmv<kTileX, kTileY, 12>), dim3(c.nt), dim3(kTileX * kTileY), 0,
-                           st, c, s, k);
+    if (c.stencil)
+        hipLaunchKernelGGL((k_cg_spmv<kTileX, kTileY, kStencil, true>), dim3(c.nt),
+                           dim3(kTileX * kTileY), 0, st, c, s, k);
+    else if (c.wp <= 12)
+        hipLaunchKernelGGL((k_cg_spmv<kTileX, kTileY, 12, false>), dim3(c.nt), dim3(kTileX * kTileY),
+                           0, st, c, s, k);
     else
-        hipLaunchKernelGGL((k_cg_spmv<kTileX, kTileY, kMaxPairWidth>), dim3(c.nt),
+        hipLaunchKernelGGL((k_cg_spmv<kTileX, kTileY, kMaxPairWidth, false>), dim3(c.nt),
                            dim3(kTileX * kTileY), 0, st, c, s, k);
     return hipGetLastError();
 }

@@ -31,6 +31,28 @@ constexpr uint16_t kEmptySlot = 0xFFFFu;
 __host__ __device__ inline int slot_di(uint16_t m) { return int(m & 15u) - 8; }
 __host__ __device__ inline int slot_dj(uint16_t m) { return int((m >> 4) & 15u) - 8; }
 
+// The reference's 12-neighbour stencil in the order its spring list numbers
+// a vertex's pairs (IE:496-530: horizontal, vertical, the two shears of each
+// quad, bend rows, bend columns): (di, dj) of the neighbour for position k.
+// A cloth whose every vertex lists its pairs as a subsequence of this order
+// (the reference's list and any sub-list of it) is described by a 12-bit
+// mask per vertex; the SpMV then takes offsets, ownership and planes as
+// constants.
+constexpr int kStencil = 12;
+__host__ __device__ constexpr int stencil_di(int k) {
+    return k == 0 ? -1 : k == 1 ? 1 : k == 4 ? -1 : k == 5 ? 1 : k == 6 ? -1 : k == 7 ? 1
+         : k == 8 ? -2 : k == 9 ? 2 : 0;
+}
+__host__ __device__ constexpr int stencil_dj(int k) {
+    return k == 2 ? -1 : k == 3 ? 1 : k == 4 ? -1 : k == 5 ? -1 : k == 6 ? 1 : k == 7 ? 1
+         : k == 10 ? -2 : k == 11 ? 2 : 0;
+}
+// The vertex owns (stores) the pair's block when the neighbour has the
+// larger index: dj > 0, or dj == 0 and di > 0 (|di| < nx for dj != 0).
+__host__ __device__ constexpr bool stencil_owner(int k) {
+    return stencil_dj(k) > 0 || (stencil_dj(k) == 0 && stencil_di(k) > 0);
+}
+
 // Grid topology in the device layout, built on the host from a spring list.
 struct GridTopology {
     int nx = 0, ny = 0, N = 0, S = 0, P = 0;
@@ -41,6 +63,9 @@ struct GridTopology {
     std::vector<int32_t> ainc_spring;              // wa*N, -1 = empty (slot-major)
     std::vector<uint16_t> ainc_meta;               // wa*N
     std::vector<uint16_t> pair_meta;               // wp*N, kEmptySlot = empty
+    bool stencil = false;                          // pairs follow the kStencil order
+    std::vector<uint16_t> pair_mask;               // N, bit k: pair at stencil position k
+    int stencil_plane[kStencil] = {};              // plane of the pair at position k
     std::vector<int32_t> pair_a, pair_b, pair_plane;
     std::vector<int32_t> spring_pair;
 };
@@ -70,6 +95,9 @@ struct DevCloth {
     const int32_t* ainc_s;
     const uint16_t* ainc_m;
     const uint16_t* pair_m;
+    const uint16_t* pair_mask;  // stencil form: bit k = pair at stencil position k
+    int stencil;                // nonzero: pairs follow the kStencil order
+    int stencil_plane[kStencil];
     const float* L0;
     const float* ks;
     const float* kd;

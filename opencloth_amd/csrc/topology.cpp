@@ -176,6 +176,37 @@ int build_grid_topology(int nx, int ny, const oc_spring* springs, int n_springs,
             slot++;
         }
     }
+
+    // stencil form: every vertex's pair slots at increasing kStencil positions
+    t.stencil = nx >= 2 && wp <= kStencil;
+    if (t.stencil) {
+        t.pair_mask.assign(N, 0);
+        for (int k = 0; k < kStencil; k++) {
+            // plane of the pair whose upper vertex is at the owner's offset
+            const int udi = stencil_owner(k) ? stencil_di(k) : -stencil_di(k);
+            const int udj = stencil_owner(k) ? stencil_dj(k) : -stencil_dj(k);
+            t.stencil_plane[k] = 0;
+            for (int q = 0; q < t.n_planes; q++)
+                if (t.di[q] == udi && t.dj[q] == udj) t.stencil_plane[k] = q;
+        }
+    }
+    for (int i = 0; i < N && t.stencil; i++) {
+        int last = -1;
+        for (int slot = 0; slot < wp; slot++) {
+            const uint16_t m = t.pair_meta[(size_t)slot * N + i];
+            if (m == kEmptySlot) break;
+            int pos = -1;
+            for (int k = 0; k < kStencil; k++)
+                if (slot_di(m) == stencil_di(k) && slot_dj(m) == stencil_dj(k)) pos = k;
+            if (pos <= last) {
+                t.stencil = false;
+                break;
+            }
+            t.pair_mask[i] |= (uint16_t)(1u << pos);
+            last = pos;
+        }
+    }
+    if (!t.stencil) t.pair_mask.clear();
     return OC_OK;
 }
 

@@ -2,7 +2,7 @@
 with the kernels' reduction order (positions, velocities, iteration count
 and status bits): the smallest grid, grids narrower or shorter than a tile,
 every vertex pinned, a spring list that is a subset of the reference's
-(only structural springs; a random subset), a NaN in the state (the
+(only structural springs; a random subset) or a permutation of it, a NaN in the state (the
 non-finite flag), and a compressed stiff cloth whose system is indefinite
 (the breakdown flag)."""
 import dataclasses
@@ -107,6 +107,18 @@ def test_spring_subsets(o32, which):
     X = scenes.perturb(sc.X, 0.02, 3)
     p = dataclasses.replace(sc.params, cg_tol=1e-7)
     compare(o32, sc.nx, sc.ny, X, np.zeros_like(X), np.ascontiguousarray(sp), sc.pinned, p, 5)
+
+
+def test_shuffled_spring_list(o32):
+    """The reference's springs in a random order: pairs are numbered by their
+    first spring, so every vertex sums its pairs in another order (the
+    kernels' general slot tables instead of the stencil mask)."""
+    sc = scenes.plain_scene()
+    perm = np.random.default_rng(5).permutation(len(sc.springs))
+    sp = np.ascontiguousarray(sc.springs[perm])
+    X = scenes.perturb(sc.X, 0.02, 9)
+    p = dataclasses.replace(sc.params, cg_tol=1e-7)
+    compare(o32, sc.nx, sc.ny, X, np.zeros_like(X), sp, sc.pinned, p, 5)
 
 
 def test_nan_state_sets_nonfinite(o32):
