@@ -159,19 +159,23 @@ def cpu_baseline(sc, budget_s: float = 12.0, single_budget_s: float = 6.0) -> di
     sysm = o.system(sc.n_vertices, sc.springs, sc.pinned)
 
     def run(threads, prm, budget):
+        """3 warm-up steps, then timed steps until `budget` seconds; the
+        rate is taken at the median step time."""
         o.set_threads(threads)
         X = o.arr(sc.X).copy(); V = o.arr(sc.V).copy(); DV = np.zeros_like(X)
-        sysm.step(X, V, DV, prm)  # warm-up
-        n = 0
+        for _ in range(3):
+            sysm.step(X, V, DV, prm)
+        times = []
         t0 = time.perf_counter()
         while True:
+            t1 = time.perf_counter()
             sysm.step(X, V, DV, prm)
-            n += 1
+            times.append(time.perf_counter() - t1)
             el = time.perf_counter() - t0
-            if el >= budget or n >= 400:
+            if el >= budget or len(times) >= 400:
                 break
         o.set_threads(1)
-        return sc.n_vertices * n / el, n, el
+        return sc.n_vertices / float(np.median(times)), len(times), el
 
     from opencloth_amd._lib import tile_shape
 
@@ -198,9 +202,9 @@ def cpu_baseline(sc, budget_s: float = 12.0, single_budget_s: float = 6.0) -> di
         "unit": "vertex-steps/s",
         "cores": nth,
         "kind": "port",
-        "sample": f"{n_mt} steps of the same {sc.nx}x{sc.ny} workload after 1 warm-up step on {nth} "
-                  f"threads (FP32 restatement, OpenMP, tile-tree dots, {el_mt:.1f} s); "
-                  f"single thread with sequential dots: {n_st} steps, {el_st:.1f} s",
+        "sample": f"median of {n_mt} timed steps of the same {sc.nx}x{sc.ny} workload after 3 warm-up "
+                  f"steps on {nth} threads (FP32 restatement, OpenMP, tile-tree dots, {el_mt:.1f} s); "
+                  f"single thread with sequential dots: median of {n_st} steps, {el_st:.1f} s",
         "single_thread_value": v_st,
         "reference_literal": {
             "value": sc.n_vertices * n_lit / el_lit,
