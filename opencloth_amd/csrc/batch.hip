@@ -43,7 +43,9 @@ __device__ inline float jacobi_inv(float dg) {
 }
 
 // Slot k of vertex i: spring id and meta, the other end's x, v (dv for a
-// warm start) and the spring's constants; sp < 0 past the last slot.
+// warm start) and the spring's constants (from the slot-major copies, so a
+// wave reads them coalesced instead of gathering by spring id); sp < 0 past
+// the last slot.
 struct BSpringIn {
     int sp;
     uint16_t m;
@@ -61,9 +63,11 @@ __device__ inline BSpringIn fetch_bspring(const DevBatch& c, const StepScalars& 
         r.x = ld3(c.x, N, j);
         r.v = ld3(c.v, N, j);
         r.dv = s.warm ? ld3(c.dv, N, j) : V3{0.0f, 0.0f, 0.0f};
-        r.L0 = c.L0[r.sp];
-        r.ks = c.ks[r.sp];
-        r.kd = c.kd[r.sp];
+        const size_t plane = (size_t)c.wa * N;
+        const size_t o = (size_t)k * N + i;
+        r.L0 = c.slot_c[o];
+        r.ks = c.slot_c[plane + o];
+        r.kd = c.slot_c[2 * plane + o];
     }
     return r;
 }
@@ -216,12 +220,14 @@ __device__ inline void adjoint_gather_vertex(const DevBatch& c, const StepScalar
                  pj = ld3(a.dvsol, N, j);
         const V3 wj = add3(vj, pj);
         const bool second = (m >> 8) & 1u;
+        const size_t plane = (size_t)c.wa * N, o = (size_t)k * N + i;
+        const float L0 = c.slot_c[o], ks = c.slot_c[plane + o], kd = c.slot_c[2 * plane + o];
         float ge[3], gvv[3], gk;
         if (second)
-            spring_adjoint(xj, xi, vj, vi, mj, mi, wj, wi, pj, pi, c.L0[sp], c.ks[sp], c.kd[sp], s.h,
+            spring_adjoint(xj, xi, vj, vi, mj, mi, wj, wi, pj, pi, L0, ks, kd, s.h,
                            s.hh, ge, gvv, gk);
         else
-            spring_adjoint(xi, xj, vi, vj, mi, mj, wi, wj, pi, pj, c.L0[sp], c.ks[sp], c.kd[sp], s.h,
+            spring_adjoint(xi, xj, vi, vj, mi, mj, wi, wj, pi, pj, L0, ks, kd, s.h,
                            s.hh, ge, gvv, gk);
         if (second) {
             gx = {gx.x - ge[0], gx.y - ge[1], gx.z - ge[2]};
@@ -391,7 +397,22 @@ __global__ void k_batch_reset(int* queue) {
     if (threadIdx.x == 0) *queue = 0;
 }
 
+__global__ __launch_bounds__(256) void k_slot_constants(DevBatch c, int q0, int nq) {
+    const size_t n = (size_t)c.wa * c.Ntot;
+    const float* src[3] = {c.L0, c.ks, c.kd};
+    for (size_t o = (size_t)blockIdx.x * 256 + threadIdx.x; o < n; o += (size_t)gridDim.x * 256) {
+        const int sp = c.ainc_s[o];
+        for (int q = q0; q < q0 + nq; q++) c.slot_c[q * n + o] = sp >= 0 ? src[q][sp] : 0.0f;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_batch_slot_constants(const DevBatch& c, int q0, int nq, hipStream_t st) {
+    if (q0 < 0 || nq < 0 || q0 + nq > 3) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_slot_constants, dim3(4096), dim3(256), 0, st, c, q0, nq);
+    return hipGetLastError();
+}
 
 hipError_t launch_batch_reset(const DevBatch& c, hipStream_t st) {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, c.queue);

@@ -264,9 +264,10 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
     const size_t am = align_up(sizeof(uint16_t) * wa * N + 4);
     const size_t pm = align_up(sizeof(uint16_t) * wp * N + 4);
     const size_t sp = align_up(sizeof(float) * S);
+    const size_t slotc = align_up(sizeof(float) * 3 * (size_t)std::max(wa, 1) * N);
     const size_t pin = align_up(N);
     const size_t per = align_up(sizeof(int) * n_cloths);
-    const size_t total = 8 * vec + blocks + as + am + pm + 3 * sp + pin + 5 * per + 256;
+    const size_t total = 8 * vec + blocks + as + am + pm + 3 * sp + slotc + pin + 5 * per + 256;
     if (hipMalloc(&b->arena, total) != hipSuccess) {
         delete b;
         return set_error(OC_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(total) + " bytes failed");
@@ -296,6 +297,7 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
     d.L0 = (const float*)take(sp);
     d.ks = (const float*)take(sp);
     d.kd = (const float*)take(sp);
+    d.slot_c = (float*)take(slotc);
     d.pin = (const uint8_t*)take(pin);
     d.inst_voff = (const int*)take(per);
     d.inst_nx = (const int*)take(per);
@@ -355,6 +357,9 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
             hipMemcpy((void*)d.inst_nx, nxv.data(), sizeof(int) * n_cloths, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy((void*)d.inst_n, nv.data(), sizeof(int) * n_cloths, hipMemcpyHostToDevice) != hipSuccess)
             return fail_free(OC_ERR_DEVICE, "upload failed");
+        if (oc::launch_batch_slot_constants(d, 0, 3, nullptr) != hipSuccess ||
+            hipDeviceSynchronize() != hipSuccess)
+            return fail_free(OC_ERR_DEVICE, "slot constants failed");
     }
     if ((rc = apply_schedule(b)) != OC_OK) {
         std::string e = oc_last_error();
@@ -553,6 +558,8 @@ int oc_batch_set_spring_stiffness(oc_batch b, const float* ks) {
     if (!b || !ks) return set_error(OC_ERR_INVALID_ARGUMENT, "NULL argument");
     OC_HIP(hipStreamSynchronize(b->stream));
     OC_HIP(hipMemcpy((void*)b->dc.ks, ks, sizeof(float) * b->Stot, hipMemcpyHostToDevice));
+    OC_HIP(oc::launch_batch_slot_constants(b->dc, 1, 1, b->stream));
+    OC_HIP(hipStreamSynchronize(b->stream));
     b->roll_valid = false;
     return OC_OK;
 }

@@ -240,6 +240,9 @@ struct DevBatch {
     const float* L0;
     const float* ks;
     const float* kd;
+    // the constants again per assembly slot, coalesced: plane q (0 L0, 1 ks,
+    // 2 kd) of slot k of vertex i at slot_c[(q * wa + k) * Ntot + i]
+    float* slot_c;
     const int* inst_voff;
     const int* inst_nx;
     const int* inst_n;
@@ -266,6 +269,8 @@ struct BatchAdjBufs {
 hipError_t launch_batch_adjoint(const DevBatch& c, const StepScalars& s, const BatchAdjBufs& a,
                                 int block_threads, int grid, hipStream_t st);
 hipError_t launch_batch_reset(const DevBatch& c, hipStream_t st);
+// Fill slot_c planes q0 .. q0+nq-1 from L0/ks/kd through the slot tables.
+hipError_t launch_batch_slot_constants(const DevBatch& c, int q0, int nq, hipStream_t st);
 hipError_t launch_batch_step(const DevBatch& c, const StepScalars& s, int block_threads,
                              int grid, hipStream_t st);
 
