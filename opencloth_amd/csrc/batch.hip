@@ -165,12 +165,38 @@ __device__ inline PairIn fetch_pair(const DevBatch& c, int i, int nx, uint16_t m
     return r;
 }
 
+// Stencil form of spmv_vertex (every cloth's pairs in the kStencil order,
+// one plane table): a 12-bit mask replaces the slot metas; the pairs are
+// summed in the same order.
+__device__ inline V3 spmv_stencil(const DevBatch& c, int i, int nx, V3 di, V3 acc) {
+    const size_t N = c.Ntot;
+    const uint32_t mask = c.pair_mask[i];
+#pragma unroll
+    for (int k = 0; k < kStencil; k++) {
+        if (!((mask >> k) & 1u)) continue;
+        const int j = i + stencil_dj(k) * nx + stencil_di(k);
+        const int blk = stencil_owner(k) ? i : j;
+        const float* bp = c.blocks + (size_t)c.stencil_plane[k] * 6 * N + blk;
+        float C[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) C[q] = bp[(size_t)q * N];
+        acc = add3(acc, symmv(C, sub3(ld3(c.d, (int)N, j), di)));
+    }
+    return acc;
+}
+
 // q_i = S (A d)_i; returns d_i . q_i.  Pair metas are loaded two slots
 // ahead and block data one slot ahead of the slot being summed.
 __device__ inline float spmv_vertex(const DevBatch& c, const StepScalars& s, int i, int nx) {
     const int N = c.Ntot;
     const V3 di = ld3(c.d, N, i);
     V3 acc = scl3(s.mass, di);
+    if (c.stencil) {
+        acc = spmv_stencil(c, i, nx, di, acc);
+        const V3 q = c.pin[i] ? V3{0.0f, 0.0f, 0.0f} : acc;
+        st3(c.q, N, i, q);
+        return dot3(di, q);
+    }
     const uint16_t m0 = c.wp > 0 ? c.pair_m[i] : kEmptySlot;
     uint16_t m1 = c.wp > 1 ? c.pair_m[(size_t)N + i] : kEmptySlot;
     PairIn cur = fetch_pair(c, i, nx, m0);
@@ -245,7 +271,7 @@ __device__ inline void adjoint_gather_vertex(const DevBatch& c, const StepScalar
 // ADJ = false: forward step.  ADJ = true: one step's adjoint at the state in
 // c.x/c.v (assembly without warm start, CG for mu in c.dv, gather).
 template <int NT, bool ADJ>
-__global__ __launch_bounds__(NT) void k_batch_step(DevBatch c, StepScalars s, BatchAdjBufs adj) {
+__global__ __launch_bounds__(NT, 4) void k_batch_step(DevBatch c, StepScalars s, BatchAdjBufs adj) {
     constexpr int NW = NT / 64;
     __shared__ float ws[NW];
     __shared__ float sd[NW], sd_old[NW], sd0[NW], ssig[NW];

@@ -257,17 +257,26 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
         wp = std::max(wp, tabs[c].topo.wp);
         b->n_planes = std::max(b->n_planes, tabs[c].topo.n_planes);
     }
+    // stencil-form SpMV when every cloth has it with the same plane table
+    bool stencil = n_cloths > 0;
+    for (int c = 0; c < n_cloths && stencil; c++) {
+        const oc::GridTopology& t = tabs[c].topo;
+        stencil = t.stencil;
+        for (int k = 0; k < oc::kStencil && stencil; k++)
+            stencil = t.stencil_plane[k] == tabs[0].topo.stencil_plane[k];
+    }
     const size_t N = b->Ntot, S = std::max(b->Stot, 1);
     const size_t vec = align_up(sizeof(float) * 3 * N);
     const size_t blocks = align_up(sizeof(float) * 6 * N * std::max(b->n_planes, 1));
     const size_t as = align_up(sizeof(int32_t) * wa * N + 4);
     const size_t am = align_up(sizeof(uint16_t) * wa * N + 4);
     const size_t pm = align_up(sizeof(uint16_t) * wp * N + 4);
+    const size_t pk = align_up(sizeof(uint16_t) * (stencil ? N : 1));
     const size_t sp = align_up(sizeof(float) * S);
     const size_t slotc = align_up(sizeof(float) * 3 * (size_t)std::max(wa, 1) * N);
     const size_t pin = align_up(N);
     const size_t per = align_up(sizeof(int) * n_cloths);
-    const size_t total = 8 * vec + blocks + as + am + pm + 3 * sp + slotc + pin + 5 * per + 256;
+    const size_t total = 8 * vec + blocks + as + am + pm + pk + 3 * sp + slotc + pin + 5 * per + 256;
     if (hipMalloc(&b->arena, total) != hipSuccess) {
         delete b;
         return set_error(OC_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(total) + " bytes failed");
@@ -294,6 +303,9 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
     d.ainc_s = (const int32_t*)take(as);
     d.ainc_m = (const uint16_t*)take(am);
     d.pair_m = (const uint16_t*)take(pm);
+    d.pair_mask = (const uint16_t*)take(pk);
+    d.stencil = stencil ? 1 : 0;
+    for (int k = 0; k < oc::kStencil; k++) d.stencil_plane[k] = stencil ? tabs[0].topo.stencil_plane[k] : 0;
     d.L0 = (const float*)take(sp);
     d.ks = (const float*)take(sp);
     d.kd = (const float*)take(sp);
@@ -318,6 +330,7 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
         std::vector<int32_t> h_as((size_t)wa * N, -1);
         std::vector<uint16_t> h_am((size_t)wa * N, oc::kEmptySlot);
         std::vector<uint16_t> h_pm((size_t)wp * N, oc::kEmptySlot);
+        std::vector<uint16_t> h_mask(stencil ? N : 0, 0);
         for (int c = 0; c < n_cloths; c++) {
             const oc::GridTopology& t = tabs[c].topo;
             const size_t off = b->voff[c];
@@ -330,10 +343,14 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
             for (int k = 0; k < t.wp; k++)
                 for (int v = 0; v < t.N; v++)
                     h_pm[(size_t)k * N + off + v] = t.pair_meta[(size_t)k * t.N + v];
+            if (stencil)
+                for (int v = 0; v < t.N; v++) h_mask[off + v] = t.pair_mask[v];
         }
         if (hipMemcpy((void*)d.ainc_s, h_as.data(), h_as.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy((void*)d.ainc_m, h_am.data(), h_am.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy((void*)d.pair_m, h_pm.data(), h_pm.size() * 2, hipMemcpyHostToDevice) != hipSuccess)
+            hipMemcpy((void*)d.pair_m, h_pm.data(), h_pm.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
+            (stencil && hipMemcpy((void*)d.pair_mask, h_mask.data(), h_mask.size() * 2,
+                                  hipMemcpyHostToDevice) != hipSuccess))
             return fail_free(OC_ERR_DEVICE, "upload of slot tables failed");
     }
     tabs.clear();

@@ -237,6 +237,9 @@ struct DevBatch {
     const int32_t* ainc_s;
     const uint16_t* ainc_m;
     const uint16_t* pair_m;
+    const uint16_t* pair_mask;  // stencil form (every cloth, one plane table)
+    int stencil;
+    int stencil_plane[kStencil];
     const float* L0;
     const float* ks;
     const float* kd;
