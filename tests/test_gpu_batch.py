@@ -52,6 +52,32 @@ def test_batch_bitwise_against_oracle(o32, sched):
         assert np.array_equal(Xg.view(np.uint32), Xr.view(np.uint32)), (c, np.abs(Xg - Xr).max())
 
 
+def test_batch_general_slot_tables(o32):
+    """One cloth's springs in a shuffled order: that cloth sums its pairs in
+    another order, so the whole batch runs on the general slot tables
+    instead of the stencil mask; bit-exact against the oracle."""
+    from oracle import make_params
+
+    from opencloth_amd.batch import ClothBatch
+
+    sc = small_batch(n=5, seed=13, sides=(16, 30))
+    n0 = int(sc.n_springs[0])
+    perm = np.random.default_rng(4).permutation(n0)
+    sc.springs[:n0] = sc.springs[:n0][perm]
+    b = ClothBatch.from_scene(sc)
+    b.set_schedule(256, 4)
+    T = b.threads_per_cloth()
+    b.step(3, graph=False)
+    X, _ = b.get_state()
+    for c in range(sc.n_cloths):
+        cs = sc.cloth(c)
+        s = o32.system(cs.n_vertices, cs.springs, cs.pinned)
+        prm = make_params(cs.params, dot_mode=2, tx=int(T[c]))
+        Xr = s.rollout(cs.X, cs.V, prm, 3)[0]
+        Xg = X[sc.voff[c]:sc.voff[c] + cs.n_vertices]
+        assert np.array_equal(Xg.view(np.uint32), Xr.view(np.uint32)), c
+
+
 def test_batch_spring_stiffness_update(o32):
     """Per-spring stiffness set after creation (the kernels read a slot-major
     copy of it) round-trips and steps every cloth bit-exactly against the
