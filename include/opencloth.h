@@ -339,8 +339,9 @@ int oc_batch_set_stream(oc_batch batch, void* hip_stream);
 int oc_batch_set_params(oc_batch batch, const oc_params* params);
 int oc_batch_set_schedule(oc_batch batch, const oc_batch_schedule* schedule);
 int oc_batch_get_schedule(oc_batch batch, oc_batch_schedule* schedule);
-/* Time every candidate schedule on one step (state saved and restored),
- * keep the fastest and return it in *best (may be NULL); synchronises. */
+/* Time every candidate schedule (the fastest of three steps from the same
+ * saved state), keep the fastest and return it in *best (may be NULL); the
+ * state is restored; synchronises. */
 int oc_batch_autotune(oc_batch batch, oc_batch_schedule* best);
 /* Threads that serve each cloth under the current schedule (n_cloths ints). */
 int oc_batch_threads_per_cloth(oc_batch batch, int32_t* threads);

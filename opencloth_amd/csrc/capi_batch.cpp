@@ -540,24 +540,27 @@ int oc_batch_autotune(oc_batch b, oc_batch_schedule* best) {
     float win_ms = 1e30f;
     int rc = snapshot(false);
     const int nts[3] = {256, 512, 1024};
-    const int vpts[5] = {2, 4, 8, 16, 32};
+    const int vpts[6] = {1, 2, 4, 8, 16, 32};
+    constexpr int kReps = 3;  // each schedule: the fastest of 3 timed steps from the same state
     for (int a = 0; a < 3 && rc == OC_OK; a++)
-        for (int v = 0; v < 5 && rc == OC_OK; v++) {
+        for (int v = 0; v < 6 && rc == OC_OK; v++) {
             b->sched = {nts[a], vpts[v]};
             if ((rc = apply_schedule(b)) != OC_OK) break;
             if ((rc = launch_step(b, b->stream)) != OC_OK) break;  // warm-up
-            if ((rc = snapshot(true)) != OC_OK) break;
-            if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = set_error(OC_ERR_DEVICE, "event"); break; }
-            if ((rc = launch_step(b, b->stream)) != OC_OK) break;
-            if (hipEventRecord(e1, b->stream) != hipSuccess) { rc = set_error(OC_ERR_DEVICE, "event"); break; }
-            if (hipEventSynchronize(e1) != hipSuccess) { rc = set_error(OC_ERR_DEVICE, "event"); break; }
-            float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            if (ms < win_ms) {
-                win_ms = ms;
-                win = b->sched;
+            for (int r = 0; r < kReps && rc == OC_OK; r++) {
+                if ((rc = snapshot(true)) != OC_OK) break;
+                if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = set_error(OC_ERR_DEVICE, "event"); break; }
+                if ((rc = launch_step(b, b->stream)) != OC_OK) break;
+                if (hipEventRecord(e1, b->stream) != hipSuccess) { rc = set_error(OC_ERR_DEVICE, "event"); break; }
+                if (hipEventSynchronize(e1) != hipSuccess) { rc = set_error(OC_ERR_DEVICE, "event"); break; }
+                float ms = 0.0f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (ms < win_ms) {
+                    win_ms = ms;
+                    win = b->sched;
+                }
             }
-            if ((rc = snapshot(true)) != OC_OK) break;
+            if (rc == OC_OK) rc = snapshot(true);
         }
     (void)hipStreamSynchronize(b->stream);
     (void)hipEventDestroy(e0);
