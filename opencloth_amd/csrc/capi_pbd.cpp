@@ -41,7 +41,12 @@ float k_prime(float k, int ns) {
 }
 
 // Greedy colouring in list order.  order gets the constraint ids colour by
-// colour, ptr the colour offsets.  Returns the colour count, -1 past 64.
+// colour, ptr the colour offsets.  Inside a colour the constraints are
+// ordered by first particle: they touch disjoint particles, so the order
+// does not change the result, and neighbouring lanes then read neighbouring
+// particles (the reference lists column constraints column by column, which
+// in list order would put a whole grid row between neighbouring lanes).
+// Returns the colour count, -1 past 64.
 int colour(int N, int n, int arity, const std::vector<int32_t>& p, std::vector<int32_t>& order,
            std::vector<int32_t>& ptr) {
     std::vector<uint64_t> used((size_t)N, 0);
@@ -62,6 +67,10 @@ int colour(int N, int n, int arity, const std::vector<int32_t>& p, std::vector<i
     std::vector<int32_t> fill(ptr.begin(), ptr.end());
     order.assign((size_t)n, 0);
     for (int c = 0; c < n; c++) order[fill[col[c]]++] = c;
+    for (int k = 0; k < ncol; k++)
+        std::stable_sort(order.begin() + ptr[k], order.begin() + ptr[k + 1], [&](int32_t x, int32_t y) {
+            return p[(size_t)arity * x] < p[(size_t)arity * y];
+        });
     return ncol;
 }
 
