@@ -121,6 +121,45 @@ def test_shuffled_spring_list(o32):
     compare(o32, sc.nx, sc.ny, X, np.zeros_like(X), sp, sc.pinned, p, 5)
 
 
+def test_no_springs_free_fall(o32):
+    """No springs at all: gravity only (free vertices fall, pins stay)."""
+    sc = scenes.plain_scene()
+    sp = np.zeros(0, scenes.SPRING_DTYPE)
+    Xg, _, _, _ = compare(o32, sc.nx, sc.ny, sc.X, np.zeros_like(sc.X), sp, sc.pinned, sc.params, 3)
+    assert np.array_equal(Xg[sc.pinned == 1], sc.X[sc.pinned == 1])
+    assert (Xg[sc.pinned == 0, 1] < sc.X[sc.pinned == 0, 1]).all()
+
+
+def test_batch_with_a_springless_cloth(o32):
+    """A batch whose middle cloth has no springs steps every cloth bit-exactly
+    against the oracle."""
+    from oracle import make_params
+
+    from opencloth_amd.batch import ClothBatch
+
+    sides = [(12, 9), (7, 5), (10, 10)]
+    Xs, sps, pins, ns = [], [], [], []
+    for c, (nx, ny) in enumerate(sides):
+        X0 = scenes.grid_positions(nx, ny, 4.0, 0.0)
+        sp = scenes.grid_springs(nx, ny, X0) if c != 1 else np.zeros(0, scenes.SPRING_DTYPE)
+        pin = np.zeros(nx * ny, np.uint8)
+        pin[:2] = 1
+        Xs.append(scenes.perturb(X0, 0.01, c)); sps.append(sp); pins.append(pin); ns.append(len(sp))
+    p = scenes.StepParams(dt=1 / 60, cg_max_iters=20, cg_tol=1e-6)
+    b = ClothBatch(np.array([a for a, _ in sides]), np.array([b_ for _, b_ in sides]), np.concatenate(Xs),
+                   np.concatenate(sps), np.array(ns), np.concatenate(pins), p)
+    b.set_schedule(256, 4)
+    T = b.threads_per_cloth()
+    b.step(3, graph=False)
+    X, _ = b.get_state()
+    off = 0
+    for c, (nx, ny) in enumerate(sides):
+        s = o32.system(nx * ny, sps[c], pins[c])
+        Xr = s.rollout(Xs[c], np.zeros_like(Xs[c]), make_params(p, dot_mode=2, tx=int(T[c])), 3)[0]
+        assert same(X[off:off + nx * ny], Xr), c
+        off += nx * ny
+
+
 def test_nan_state_sets_nonfinite(o32):
     """A NaN position: the status carries OC_SOLVE_NONFINITE, as in the
     oracle, and the same entries are NaN afterwards."""
