@@ -25,6 +25,8 @@ def hip() -> C.CDLL:
         _hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
         _hip.hipEventDestroy.argtypes = [C.c_void_p]
         _hip.hipStreamSynchronize.argtypes = [C.c_void_p]
+        _hip.hipStreamCreateWithFlags.argtypes = [C.POINTER(C.c_void_p), C.c_uint]
+        _hip.hipStreamDestroy.argtypes = [C.c_void_p]
         _hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
         _hip.hipFree.argtypes = [C.c_void_p]
         _hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
@@ -58,6 +60,30 @@ class Event:
     def __del__(self):
         try:
             hip().hipEventDestroy(self.e)
+        except Exception:
+            pass
+
+
+class Stream:
+    """A non-blocking HIP stream at the default priority (for handing to
+    oc_*_set_stream); .handle is the hipStream_t."""
+
+    def __init__(self):
+        self.s = C.c_void_p()
+        rc = hip().hipStreamCreateWithFlags(C.byref(self.s), 1)  # hipStreamNonBlocking
+        if rc != 0:
+            raise RuntimeError(f"hipStreamCreateWithFlags failed ({rc})")
+
+    @property
+    def handle(self) -> int:
+        return self.s.value or 0
+
+    def synchronize(self):
+        hip().hipStreamSynchronize(self.s)
+
+    def __del__(self):
+        try:
+            hip().hipStreamDestroy(self.s)
         except Exception:
             pass
 
