@@ -9,9 +9,11 @@
 // drained, which every workgroup reaches.
 //
 // Per cloth the arithmetic is the single-cloth step's (same per-vertex
-// gathers, pair order and expressions, optional Jacobi preconditioner).  Reductions: thread t of a cloth's T
-// threads sums vertices t, t+T, ... in order, then the 64-lane wave tree,
-// then the cloth's waves in order.
+// gathers, pair order and expressions, optional Jacobi preconditioner).
+// Spring constants come from slot-major copies and, when every cloth has
+// the stencil form, the SpMV takes its neighbours from a 12-bit mask.
+// Reductions: thread t of a cloth's T threads sums vertices t, t+T, ... in
+// order, then the 64-lane wave tree, then the cloth's waves in order.
 #include "oc_device.hpp"
 
 namespace oc {
