@@ -66,7 +66,7 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
-HOST_SRCS = [os.path.join(HERE, "host", "ie_demo.cpp")]
+HOST_SRCS = [os.path.join(HERE, "host", f) for f in ("ie_demo.cpp", "vl_demo.cpp", "pbd_demo.cpp")]
 
 
 def host_program(name: str) -> str:
@@ -87,7 +87,8 @@ def build_host(verbose: bool = False) -> list:
     for s in HOST_SRCS:
         exe = os.path.join(LIBDIR, os.path.splitext(os.path.basename(s))[0])
         cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-ffp-contract=off", "-Wall",
-               "-I", os.path.join(ROOT, "include"), s, "-o", exe + ".tmp", "-L", LIBDIR, "-lopencloth",
+               "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host"), s, "-o", exe + ".tmp",
+               "-L", LIBDIR, "-lopencloth",
                "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
         if verbose:
             print(" ".join(cmd), flush=True)

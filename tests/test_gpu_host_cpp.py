@@ -1,7 +1,7 @@
-"""The C++ demo program (opencloth_amd/host/ie_demo.cpp: the implicit demo's
-scene and StepPhysics loop through include/opencloth.hpp over the C ABI)
-against the FP32 oracle on the same scene, bit for bit, after enough steps
-for the cloth to reach the collider."""
+"""The C++ demo programs (opencloth_amd/host/{ie,vl,pbd}_demo.cpp: each
+reference demo's scene and StepPhysics loop through include/opencloth.hpp
+over the C ABI) against the FP32 oracle on the same scene, bit for bit, after
+enough steps for the cloth to reach the collider."""
 import struct
 import subprocess
 
@@ -38,5 +38,49 @@ def test_cpp_demo_matches_oracle(tmp_path):
     for _ in range(steps):
         s.step_physics(X, V, DV, prm, ell, slack)
     assert X[:, 1].min() < 2.5  # in contact with the collider
+    assert np.array_equal(Xg.view(np.uint32), X.view(np.uint32))
+    assert np.array_equal(Vg.view(np.uint32), V.view(np.uint32))
+
+
+def run_demo(name, steps, tmp_path):
+    out = tmp_path / "state.bin"
+    say(name, steps)
+    r = subprocess.run([host_program(name), "--steps", str(steps), "--out", str(out)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    say(r.stdout.strip())
+    return open(out, "rb").read()
+
+
+def test_cpp_verlet_demo_matches_oracle(tmp_path):
+    from oracle import Oracle
+
+    steps = 300
+    b = run_demo("vl_demo", steps, tmp_path)
+    n, _ = struct.unpack("<ii", b[:8])
+    Xg = np.frombuffer(b, np.float32, 3 * n, 8).reshape(n, 3)
+    XLg = np.frombuffer(b, np.float32, 3 * n, 8 + 12 * n).reshape(n, 3)
+    sc = scenes.verlet_demo_scene()
+    p = sc.params
+    s = Oracle(32).system(sc.n_vertices, sc.springs, sc.pinned)
+    Xr, XLr = s.verlet_rollout(sc.X, sc.X, p.dt, p.mass, p.gravity, sc.extra["damping"], 1, steps,
+                               sc.extra["ellipsoid"])
+    assert np.array_equal(Xg.view(np.uint32), Xr.view(np.uint32))
+    assert np.array_equal(XLg.view(np.uint32), XLr.view(np.uint32))
+
+
+def test_cpp_pbd_demo_matches_oracle(tmp_path):
+    from oracle import Oracle, make_pbd_params
+
+    steps = 600
+    b = run_demo("pbd_demo", steps, tmp_path)
+    n, _, _ = struct.unpack("<iii", b[:12])
+    Xg = np.frombuffer(b, np.float32, 3 * n, 12).reshape(n, 3)
+    Vg = np.frombuffer(b, np.float32, 3 * n, 12 + 12 * n).reshape(n, 3)
+    sc = scenes.pbd_demo_scene()
+    o = Oracle(32)
+    s = o.pbd(sc.X, sc.pinned, sc.params.mass, sc.distance, sc.bending, sc.params.solver_iterations)
+    X, V = s.step(sc.X, sc.V, make_pbd_params(sc.params), 1, steps, sc.extra["ellipsoid"])
+    assert X[:, 1].min() < 2.0  # draped on the collider
     assert np.array_equal(Xg.view(np.uint32), X.view(np.uint32))
     assert np.array_equal(Vg.view(np.uint32), V.view(np.uint32))

@@ -1,8 +1,8 @@
-"""The C++ host layer (include/opencloth.hpp) and the implicit demo built on
-it (opencloth_amd/host/ie_demo.cpp): the program's scene, made by its own
-C++ restatement of InitGL (IE:454-544), equals the Python demo scene bit for
-bit, so the GPU test of its StepPhysics loop runs the oracle on the same
-inputs (CPU only: --scene-only makes no GPU call)."""
+"""The C++ host layer (include/opencloth.hpp) and the demos built on it
+(opencloth_amd/host/{ie,vl,pbd}_demo.cpp): each program's scene, made by its
+own C++ restatement of the reference demo's InitGL, equals the Python demo
+scene bit for bit, so the GPU tests of their StepPhysics loops run the
+oracle on the same inputs (CPU only: --scene-only makes no GPU call)."""
 import struct
 import subprocess
 
@@ -38,6 +38,43 @@ def test_demo_scene_matches(tmp_path):
     assert np.array_equal(ell[:16].view(np.uint32), e["M"].view(np.uint32))
     assert np.array_equal(ell[16:32].view(np.uint32), e["Minv"].view(np.uint32))
     assert np.array_equal(ell[32:35], e["center"]) and ell[35] == e["radius"]
+
+
+def test_verlet_demo_scene_matches(tmp_path):
+    out = tmp_path / "scene.bin"
+    r = subprocess.run([host_program("vl_demo"), "--scene-only", "--out", str(out)],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    X, springs, pinned, ell = read_scene(out)
+    sc = scenes.verlet_demo_scene()
+    e = sc.extra["ellipsoid"]
+    assert np.array_equal(X.view(np.uint32), sc.X.view(np.uint32))
+    assert np.array_equal(springs.view(np.uint8), sc.springs.view(np.uint8))
+    assert np.array_equal(pinned, sc.pinned)
+    assert np.array_equal(ell[:16].view(np.uint32), e["M"].view(np.uint32))
+    assert np.array_equal(ell[16:32].view(np.uint32), e["Minv"].view(np.uint32))
+
+
+def test_pbd_demo_scene_matches(tmp_path):
+    out = tmp_path / "scene.bin"
+    r = subprocess.run([host_program("pbd_demo"), "--scene-only", "--out", str(out)],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    b = open(out, "rb").read()
+    n, nd, nb = struct.unpack("<iii", b[:12])
+    off = 12
+    X = np.frombuffer(b, np.float32, 3 * n, off).reshape(n, 3); off += 12 * n
+    d = np.frombuffer(b, scenes.PBD_DISTANCE_DTYPE, nd, off); off += 12 * nd
+    bend = np.frombuffer(b, scenes.PBD_BENDING_DTYPE, nb, off); off += 16 * nb
+    pinned = np.frombuffer(b, np.uint8, n, off); off += n
+    ell = np.frombuffer(b, np.float32, 36, off); off += 144
+    assert off == len(b)
+    sc = scenes.pbd_demo_scene()
+    assert np.array_equal(X.view(np.uint32), sc.X.view(np.uint32))
+    assert np.array_equal(d.view(np.uint8), sc.distance.view(np.uint8))
+    assert np.array_equal(bend.view(np.uint8), sc.bending.view(np.uint8))
+    assert np.array_equal(pinned, sc.pinned)
+    assert np.array_equal(ell[16:32].view(np.uint32), sc.extra["ellipsoid"]["Minv"].view(np.uint32))
 
 
 def test_demo_rejects_bad_arguments(tmp_path):
