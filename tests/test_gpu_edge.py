@@ -121,6 +121,25 @@ def test_shuffled_spring_list(o32):
     compare(o32, sc.nx, sc.ny, X, np.zeros_like(X), sp, sc.pinned, p, 5)
 
 
+def test_springs_off_the_stencil(o32):
+    """Springs two columns and one row apart (outside the reference's 12
+    neighbours, inside the 2-wide halo) appended to the reference list:
+    the general slot tables, bit-exact."""
+    sc = scenes.plain_scene()
+    n = sc.nx
+    extra = np.zeros(30, scenes.SPRING_DTYPE)
+    rows = np.arange(30) % (n - 1)
+    cols = (np.arange(30) * 7) % (n - 2)
+    extra["p1"] = rows * n + cols
+    extra["p2"] = (rows + 1) * n + cols + 2
+    extra["rest_length"] = scenes.rest_lengths(sc.X, extra["p1"], extra["p2"])
+    extra["Ks"], extra["Kd"], extra["type"] = 30.0, -0.25, 2
+    sp = np.concatenate([sc.springs, extra])
+    X = scenes.perturb(sc.X, 0.02, 17)
+    p = dataclasses.replace(sc.params, cg_tol=1e-7)
+    compare(o32, sc.nx, sc.ny, X, np.zeros_like(X), sp, sc.pinned, p, 4)
+
+
 def test_no_springs_free_fall(o32):
     """No springs at all: gravity only (free vertices fall, pins stay)."""
     sc = scenes.plain_scene()
