@@ -345,6 +345,9 @@ int oc_batch_get_schedule(oc_batch batch, oc_batch_schedule* schedule);
 int oc_batch_autotune(oc_batch batch, oc_batch_schedule* best);
 /* Threads that serve each cloth under the current schedule (n_cloths ints). */
 int oc_batch_threads_per_cloth(oc_batch batch, int32_t* threads);
+/* Copy positions/velocities of all cloths in (host AoS, vertex offsets in
+ * cloth order; either may be NULL, which leaves it unchanged).  New
+ * positions also clear the CG warm start, so the next solve starts at zero. */
 int oc_batch_set_state(oc_batch batch, const float* x, const float* v);
 int oc_batch_get_state(oc_batch batch, float* x, float* v);
 /* nsteps steps, each a replay of the captured HIP graph of one step. */

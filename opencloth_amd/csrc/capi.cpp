@@ -87,31 +87,34 @@ int check_params(const oc_params* p) {
     return OC_OK;
 }
 
-// (Re)allocate the CG partials for the current iteration cap.
-int ensure_scratch(oc_cloth_s* c) {
-    const int need = std::max(c->prm.cg_max_iters, 1);
+// (Re)allocate the CG partials for an iteration cap of max_iters.  The new
+// buffer is allocated before the old one is released, so on failure the
+// handle keeps working with its previous cap.
+int ensure_scratch(oc_cloth_s* c, int max_iters) {
+    const int need = std::max(max_iters, 1);
     if (c->scratch && need <= c->max_it_alloc) return OC_OK;
-    if (c->scratch) {
-        OC_HIP(hipStreamSynchronize(c->stream));
-        OC_HIP(hipFree(c->scratch));
-        c->scratch = nullptr;
-    }
     const size_t nt = c->dc.nt;
-    const size_t bytes = align_up(sizeof(float) * (need + 1) * nt) +
-                         align_up(sizeof(float) * need * nt) + align_up(sizeof(float) * (need + 2)) +
-                         align_up(sizeof(int) * 8);
-    if (hipMalloc(&c->scratch, bytes) != hipSuccess)
+    const size_t del = align_up(sizeof(float) * (need + 1) * nt), sig = align_up(sizeof(float) * need * nt),
+                 scal = align_up(sizeof(float) * (need + 2));
+    const size_t bytes = del + sig + scal + align_up(sizeof(int) * 8);
+    void* fresh = nullptr;
+    if (hipMalloc(&fresh, bytes) != hipSuccess)
         return fail(OC_ERR_OUT_OF_MEMORY, "hipMalloc of CG scratch failed");
-    char* p = (char*)c->scratch;
+    if (c->scratch) {
+        if (hipStreamSynchronize(c->stream) != hipSuccess) {
+            (void)hipFree(fresh);
+            return fail(OC_ERR_DEVICE, "hipStreamSynchronize failed");
+        }
+        (void)hipFree(c->scratch);
+    }
+    c->scratch = fresh;
+    char* p = (char*)fresh;
     c->dc.del_part = (float*)p;
-    p += align_up(sizeof(float) * (need + 1) * nt);
-    c->dc.sig_part = (float*)p;
-    p += align_up(sizeof(float) * need * nt);
-    c->dc.scal = (float*)p;
-    p += align_up(sizeof(float) * (need + 2));
-    c->dc.ctl = (int*)p;
-    OC_HIP(hipMemsetAsync(c->scratch, 0, bytes, c->stream));
+    c->dc.sig_part = (float*)(p + del);
+    c->dc.scal = (float*)(p + del + sig);
+    c->dc.ctl = (int*)(p + del + sig + scal);
     c->max_it_alloc = need;
+    OC_HIP(hipMemsetAsync(fresh, 0, bytes, c->stream));
     return OC_OK;
 }
 
@@ -305,7 +308,7 @@ int oc_cloth_create(int32_t nx, int32_t ny, const float* x0, const float* v0,
     if (!ok) return cleanup(fail(OC_ERR_DEVICE, "upload of topology failed"));
     if (hipHostMalloc((void**)&c->ctl_host, sizeof(int) * 8) != hipSuccess)
         return cleanup(fail(OC_ERR_OUT_OF_MEMORY, "hipHostMalloc failed"));
-    if ((rc = ensure_scratch(c)) != OC_OK) return cleanup(rc);
+    if ((rc = ensure_scratch(c, params->cg_max_iters)) != OC_OK) return cleanup(rc);
     if ((rc = upload_vec(c, d.x, x0)) != OC_OK) return cleanup(rc);
     if ((rc = upload_vec(c, d.v, v0)) != OC_OK) return cleanup(rc);
     *out = c;
@@ -333,9 +336,10 @@ int oc_cloth_set_params(oc_cloth c, const oc_params* p) {
     if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
     int rc = check_params(p);
     if (rc != OC_OK) return rc;
+    if ((rc = ensure_scratch(c, p->cg_max_iters)) != OC_OK) return rc;  // old params stay on failure
     c->prm = *p;
     c->dc.persistent = (p->flags & OC_FLAG_CG_PERSISTENT) ? oc::persistent_cg_grid(c->dc) : 0;
-    return ensure_scratch(c);
+    return OC_OK;
 }
 
 int oc_cloth_get_params(oc_cloth c, oc_params* p) {
@@ -655,6 +659,7 @@ extern "C" {
 int oc_cloth_rollout_forward(oc_cloth c, int32_t nsteps, int32_t every) {
     if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
     if (nsteps < 1 || every < 1) return fail(OC_ERR_INVALID_ARGUMENT, "nsteps and checkpoint_every must be >= 1");
+    c->roll_valid = false;  // until this rollout has completed
     every = std::min(every, nsteps);
     const int nseg = (nsteps + every - 1) / every;
     const size_t v3 = 3 * (size_t)c->dc.N;

@@ -103,53 +103,59 @@ int apply_schedule(oc_batch_s* b) {
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
                      [&](int x, int y) { return b->nverts[x] > b->nverts[y]; });
-    b->T_of.assign(b->n, 64);
+    std::vector<int> T_of(b->n, 64);
     for (int c = 0; c < b->n; c++) {
         const int want = (b->nverts[c] + b->sched.vertices_per_thread - 1) / b->sched.vertices_per_thread;
-        b->T_of[c] = std::min(NT, std::max(64, pow2ceil(want)));
+        T_of[c] = std::min(NT, std::max(64, pow2ceil(want)));
     }
     std::vector<int> it_inst, it_count, it_T;
     for (int k = 0; k < b->n;) {
-        const int T = b->T_of[order[k]];
+        const int T = T_of[order[k]];
         const int G = NT / T;
         int cnt = 0;
-        while (k + cnt < b->n && cnt < G && b->T_of[order[k + cnt]] == T) cnt++;
+        while (k + cnt < b->n && cnt < G && T_of[order[k + cnt]] == T) cnt++;
         it_inst.push_back(k);
         it_count.push_back(cnt);
         it_T.push_back(T);
         k += cnt;
     }
     const int n_items = (int)it_inst.size();
-    if (b->sched_buf) {
-        OC_HIP(hipStreamSynchronize(b->stream));
-        OC_HIP(hipFree(b->sched_buf));
-        b->sched_buf = nullptr;
-    }
+    // persistent grid: as many workgroups as fit on the device at once
+    int dev = 0, cus = 0;
+    OC_HIP(hipGetDevice(&dev));
+    OC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int grid = std::max(1, std::min(n_items, cus * std::max(1, 2048 / NT)));
+    // the new tables go to a new buffer; the old one (which a captured step
+    // may still name) is released only once the new one is in place
     const size_t bi = align_up(sizeof(int) * std::max(n_items, 1));
     const size_t bo = align_up(sizeof(int) * std::max(b->n, 1));
-    if (hipMalloc(&b->sched_buf, 3 * bi + bo) != hipSuccess)
+    void* fresh = nullptr;
+    if (hipMalloc(&fresh, 3 * bi + bo) != hipSuccess)
         return set_error(OC_ERR_OUT_OF_MEMORY, "hipMalloc of the schedule failed");
-    char* p = (char*)b->sched_buf;
+    char* p = (char*)fresh;
     int* d_inst = (int*)p;
     int* d_count = (int*)(p + bi);
     int* d_T = (int*)(p + 2 * bi);
     int* d_order = (int*)(p + 3 * bi);
-    OC_HIP(hipMemcpy(d_inst, it_inst.data(), sizeof(int) * n_items, hipMemcpyHostToDevice));
-    OC_HIP(hipMemcpy(d_count, it_count.data(), sizeof(int) * n_items, hipMemcpyHostToDevice));
-    OC_HIP(hipMemcpy(d_T, it_T.data(), sizeof(int) * n_items, hipMemcpyHostToDevice));
-    OC_HIP(hipMemcpy(d_order, order.data(), sizeof(int) * b->n, hipMemcpyHostToDevice));
+    const bool ok =
+        hipMemcpy(d_inst, it_inst.data(), sizeof(int) * n_items, hipMemcpyHostToDevice) == hipSuccess &&
+        hipMemcpy(d_count, it_count.data(), sizeof(int) * n_items, hipMemcpyHostToDevice) == hipSuccess &&
+        hipMemcpy(d_T, it_T.data(), sizeof(int) * n_items, hipMemcpyHostToDevice) == hipSuccess &&
+        hipMemcpy(d_order, order.data(), sizeof(int) * b->n, hipMemcpyHostToDevice) == hipSuccess &&
+        (!b->sched_buf || hipStreamSynchronize(b->stream) == hipSuccess);
+    if (!ok) {
+        (void)hipFree(fresh);
+        return set_error(OC_ERR_DEVICE, "upload of the schedule failed");
+    }
+    if (b->sched_buf) (void)hipFree(b->sched_buf);
+    b->sched_buf = fresh;
+    b->T_of.swap(T_of);
     b->dc.item_inst = d_inst;
     b->dc.item_count = d_count;
     b->dc.item_T = d_T;
     b->dc.order = d_order;
     b->dc.n_items = n_items;
-
-    // persistent grid: as many workgroups as fit on the device at once
-    int dev = 0, cus = 0, per_cu = 0;
-    OC_HIP(hipGetDevice(&dev));
-    OC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    per_cu = std::max(1, 2048 / NT);
-    b->grid = std::max(1, std::min(n_items, cus * per_cu));
+    b->grid = grid;
     b->graph_dirty = true;
     return OC_OK;
 }
@@ -425,13 +431,8 @@ int oc_batch_set_schedule(oc_batch b, const oc_batch_schedule* s) {
     oc_batch_schedule old = b->sched;
     b->sched = *s;
     int rc = apply_schedule(b);
-    if (rc != OC_OK) {
-        b->sched = old;
-        std::string e = oc_last_error();
-        apply_schedule(b);
-        return set_error(rc, e);
-    }
-    return OC_OK;
+    if (rc != OC_OK) b->sched = old;  // the old tables are still in place
+    return rc;
 }
 
 int oc_batch_get_schedule(oc_batch b, oc_batch_schedule* s) {
@@ -450,16 +451,21 @@ int oc_batch_set_state(oc_batch b, const float* x, const float* v) {
     if (!b) return set_error(OC_ERR_INVALID_ARGUMENT, "batch is NULL");
     const size_t N = b->Ntot;
     std::vector<float> t(3 * N);
+    // ordered on the batch's stream; t is reused, so each copy completes first
     auto up = [&](float* dst, const float* src) -> int {
         for (size_t i = 0; i < N; i++)
-            for (int k = 0; k < 3; k++) t[k * N + i] = src ? src[3 * i + k] : 0.0f;
-        OC_HIP(hipMemcpy(dst, t.data(), t.size() * 4, hipMemcpyHostToDevice));
+            for (int k = 0; k < 3; k++) t[k * N + i] = src[3 * i + k];
+        OC_HIP(hipMemcpyAsync(dst, t.data(), t.size() * 4, hipMemcpyHostToDevice, b->stream));
+        OC_HIP(hipStreamSynchronize(b->stream));
         return OC_OK;
     };
     int rc;
     if (x && (rc = up(b->dc.x, x)) != OC_OK) return rc;
-    if ((x || v) && (rc = up(b->dc.v, v)) != OC_OK) return rc;
-    if (x) OC_HIP(hipMemset(b->dc.dv, 0, sizeof(float) * 3 * N));
+    if (v && (rc = up(b->dc.v, v)) != OC_OK) return rc;
+    if (x) {
+        OC_HIP(hipMemsetAsync(b->dc.dv, 0, sizeof(float) * 3 * N, b->stream));
+        OC_HIP(hipStreamSynchronize(b->stream));
+    }
     return OC_OK;
 }
 
@@ -674,6 +680,7 @@ int oc_batch_rollout_forward(oc_batch b, int32_t nsteps, int32_t every) {
     if (!b) return set_error(OC_ERR_INVALID_ARGUMENT, "batch is NULL");
     if (nsteps < 1 || every < 1)
         return set_error(OC_ERR_INVALID_ARGUMENT, "nsteps and checkpoint_every must be >= 1");
+    b->roll_valid = false;  // until this rollout has completed
     every = std::min(every, nsteps);
     const int nseg = (nsteps + every - 1) / every;
     const size_t v3 = 3 * (size_t)b->Ntot;

@@ -78,6 +78,41 @@ def test_batch_general_slot_tables(o32):
         assert np.array_equal(Xg.view(np.uint32), Xr.view(np.uint32)), c
 
 
+def test_batch_rejected_schedule_and_partial_state():
+    """A rejected schedule leaves the previous one in force (same threads
+    per cloth, a captured step still replays bit-exactly); set_state with
+    one of x and v leaves the other unchanged."""
+    from opencloth_amd._lib import OpenClothError
+    from opencloth_amd.batch import ClothBatch
+
+    sc = small_batch(n=6, seed=5, sides=(16, 32))
+    a = ClothBatch.from_scene(sc)
+    b = ClothBatch.from_scene(sc)
+    a.set_schedule(512, 4)
+    b.set_schedule(512, 4)
+    T = a.threads_per_cloth()
+    a.step(1, graph=True)  # captures a step with the 512/4 tables
+    b.step(1, graph=True)
+    for bad in ((384, 4), (512, 0)):
+        with pytest.raises(OpenClothError):
+            a.set_schedule(*bad)
+        assert np.array_equal(a.threads_per_cloth(), T)
+    a.step(2, graph=True)
+    b.step(2, graph=True)
+    Xa, Va = a.get_state()
+    Xb, Vb = b.get_state()
+    assert np.array_equal(Xa.view(np.uint32), Xb.view(np.uint32))
+    assert np.array_equal(Va.view(np.uint32), Vb.view(np.uint32))
+    a.set_state(V=np.zeros_like(Va))
+    X1, V1 = a.get_state()
+    assert np.array_equal(X1, Xa) and not V1.any()
+    a.set_state(X=Xa + 1.0)
+    X2, V2 = a.get_state()
+    assert np.array_equal(X2, Xa + 1.0) and not V2.any()
+    a.close()
+    b.close()
+
+
 def test_batch_spring_stiffness_update(o32):
     """Per-spring stiffness set after creation (the kernels read a slot-major
     copy of it) round-trips and steps every cloth bit-exactly against the
