@@ -220,7 +220,7 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
     std::vector<int64_t> soff(n_cloths + 1, 0);
     int64_t Nt = 0;
     for (int c = 0; c < n_cloths; c++) {
-        if (nx[c] < 1 || ny[c] < 1 || n_springs[c] < 0) {
+        if (nx[c] < 1 || ny[c] < 1 || n_springs[c] < 0 || (int64_t)nx[c] * ny[c] > (1 << 28)) {
             delete b;
             return set_error(OC_ERR_INVALID_ARGUMENT, "invalid cloth size");
         }

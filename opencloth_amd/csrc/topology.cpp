@@ -24,6 +24,10 @@ int build_grid_topology(int nx, int ny, const oc_spring* springs, int n_springs,
         *err = "invalid spring list";
         return OC_ERR_INVALID_ARGUMENT;
     }
+    if (n_springs > (1 << 29)) {  // the incidence offsets below count 2 per spring in int
+        *err = "too many springs";
+        return OC_ERR_INVALID_ARGUMENT;
+    }
     GridTopology& t = *out;
     t = GridTopology();
     t.nx = nx;
