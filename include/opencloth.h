@@ -223,6 +223,48 @@ int oc_verlet_compute_forces(oc_verlet verlet, float* f_out);
  * Replaces StepPhysics of the Verlet demo (VL:557-562). */
 int oc_verlet_step(oc_verlet verlet, int32_t nsteps, const oc_ellipsoid* ellipsoid);
 
+/* ---- grid Verlet: the GPU demos' kernel ---------------------------------- */
+/* CUK = OpenCloth_Verlet_CUDA_GLSL_OPENCL_CPU/OpenCloth_Verlet_CUDA/verlet_kernel.cu,
+ * CUH = .../verlet.cu (its host glue), CUM = .../main.cpp (the caller).
+ * One vertex per texel of a width x height grid, a fixed 12-neighbour
+ * stencil (structural +-1, shear diagonals, bend +-2; CUK:14-64) with rest
+ * lengths from the grid spacing (CUK:164) and one ks/kd per class, no spring
+ * list.  Positions are host AoS float4 (x, y, z, w), as the demo's
+ * glm::vec4 arrays (CUM:441); w is carried unchanged. */
+typedef struct oc_vgrid_s* oc_vgrid;
+
+typedef struct {
+    float mass;        /* CUM:100 */
+    float gravity[3];  /* acceleration; the kernel's is (0, -0.00981, 0) (CUK:123) */
+    float damping;     /* DEFAULT_DAMPING (CUM:95) */
+    float dt;          /* timeStep (CUM:102) */
+    float spacing[2];  /* inv_cloth_size = (sizeX/numX, sizeY/numY) (CUM:601) */
+    float ks[3];       /* structural, shear, bend (CUK:8-10) */
+    float kd[3];       /* the same classes; the demo's are negative (CUK:8-10) */
+} oc_vgrid_params;
+
+/* Allocates the device state and uploads x0 and x_last0 (width*height
+ * float4 each; x_last0 may be NULL: start at rest).  pinned: one byte per
+ * vertex, non-zero = zero acceleration; NULL pins vertices 0 and width-1 as
+ * the kernel does (CUK:126-127).  Replaces InitCUDA (CUH:16-25) plus the
+ * first UploadCUDA (CUH:53-79). */
+int oc_vgrid_create(int32_t width, int32_t height, const float* x0, const float* x_last0,
+                    const uint8_t* pinned, const oc_vgrid_params* params, oc_vgrid* out);
+/* Replaces ShutdownCUDA (CUH:27-45). */
+int oc_vgrid_destroy(oc_vgrid grid);
+int oc_vgrid_set_stream(oc_vgrid grid, void* hip_stream);
+int oc_vgrid_set_params(oc_vgrid grid, const oc_vgrid_params* params);
+/* Host AoS float4 in and out (either may be NULL); get synchronises. */
+int oc_vgrid_set_state(oc_vgrid grid, const float* x, const float* x_last);
+int oc_vgrid_get_state(oc_vgrid grid, float* x, float* x_last);
+/* nsteps of the kernel's step (CUK:70-197): force = gravity*mass +
+ * damping*v plus the stencil's springs, ground clamp of the current
+ * position, x' = 2x - x_last + a dt^2, x_last' = x.  vbo: NULL or a device
+ * array of width*height float4 that receives the last step's positions, as
+ * the mapped vertex buffer of VerletCUDA (CUH:81-100, CUM:597-607).  Stream
+ * ordered. */
+int oc_vgrid_step(oc_vgrid grid, int32_t nsteps, float* vbo);
+
 /* ---- position-based dynamics (PBD) --------------------------------------- */
 /* PBD = OpenCloth_PositionBasedDynamics/OpenCloth_PositionBasedDynamics/main.cpp,
  * built with USE_TRIANGLE_BENDING_CONSTRAINT (PBD:57). */

@@ -67,6 +67,13 @@ SIGNATURES = {
     "oc_verlet_get_state": (C.c_int, [_P, _P, _P]),
     "oc_verlet_compute_forces": (C.c_int, [_P, _P]),
     "oc_verlet_step": (C.c_int, [_P, _I, _P]),
+    "oc_vgrid_create": (C.c_int, [_I, _I, _P, _P, _P, _P, _P]),
+    "oc_vgrid_destroy": (C.c_int, [_P]),
+    "oc_vgrid_set_stream": (C.c_int, [_P, _P]),
+    "oc_vgrid_set_params": (C.c_int, [_P, _P]),
+    "oc_vgrid_set_state": (C.c_int, [_P, _P, _P]),
+    "oc_vgrid_get_state": (C.c_int, [_P, _P, _P]),
+    "oc_vgrid_step": (C.c_int, [_P, _I, _P]),
     "oc_pbd_create": (C.c_int, [_I, _P, _P, _P, _I, _P, _I, _P, _P]),
     "oc_pbd_destroy": (C.c_int, [_P]),
     "oc_pbd_set_stream": (C.c_int, [_P, _P]),
@@ -129,6 +136,25 @@ class OcVerletParams(C.Structure):
 
     _fields_ = [("mass", C.c_float), ("gravity", C.c_float * 3), ("damping", C.c_float),
                 ("dt", C.c_float), ("flags", C.c_int32)]
+
+
+class OcVgridParams(C.Structure):
+    """oc_vgrid_params (include/opencloth.h)."""
+
+    _fields_ = [("mass", C.c_float), ("gravity", C.c_float * 3), ("damping", C.c_float),
+                ("dt", C.c_float), ("spacing", C.c_float * 2), ("ks", C.c_float * 3),
+                ("kd", C.c_float * 3)]
+
+    @classmethod
+    def from_dict(cls, p: dict) -> "OcVgridParams":
+        o = cls()
+        o.mass, o.damping, o.dt = p["mass"], p["damping"], p["dt"]
+        for k in range(3):
+            o.gravity[k] = p["gravity"][k]
+            o.ks[k] = p["ks"][k]
+            o.kd[k] = p["kd"][k]
+        o.spacing[0], o.spacing[1] = p["spacing"]
+        return o
 
 
 class OcPbdParams(C.Structure):

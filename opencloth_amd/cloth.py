@@ -13,7 +13,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import OcEllipsoid, OcParams, OcPbdParams, OcVerletParams, check, lib, ptr
+from ._lib import OcEllipsoid, OcParams, OcPbdParams, OcVerletParams, OcVgridParams, check, lib, ptr
 from .scenes import (PBD_BENDING_DTYPE, PBD_DISTANCE_DTYPE, SPRING_DTYPE, PBDParams, Scene,
                      StepParams)
 
@@ -277,6 +277,74 @@ class VerletCloth:
         """StepPhysics of the Verlet demo (VL:557-562)."""
         e = None if ell is None else OcEllipsoid.from_dict(ell)
         check(lib().oc_verlet_step(self.h, int(nsteps), C.byref(e) if e is not None else None))
+
+
+class VerletGrid:
+    """Grid Verlet (oc_vgrid): the GPU demos' Verlet kernel
+    (OpenCloth_Verlet_CUDA_GLSL_OPENCL_CPU/OpenCloth_Verlet_CUDA/verlet_kernel.cu:70-197)
+    on a width x height grid of float4 positions, 12-neighbour stencil."""
+
+    def __init__(self, width, height, X, X_last=None, pinned=None, params: dict | None = None):
+        from .scenes import vgrid_params
+
+        self.width, self.height = int(width), int(height)
+        self.N = self.width * self.height
+        self.params = dict(params or vgrid_params(self.width, self.height))
+        self._p = OcVgridParams.from_dict(self.params)
+        X4 = self._vec4(X)
+        XL4 = None if X_last is None else self._vec4(X_last)
+        pins = None if pinned is None else np.ascontiguousarray(pinned, np.uint8)
+        if pins is not None and pins.shape != (self.N,):
+            raise ValueError("pinned must have one entry per vertex")
+        h = C.c_void_p()
+        check(lib().oc_vgrid_create(self.width, self.height, ptr(X4), ptr(XL4), ptr(pins),
+                                    C.byref(self._p), C.byref(h)))
+        self.h = h
+
+    @classmethod
+    def from_scene(cls, sc) -> "VerletGrid":
+        return cls(sc.width, sc.height, sc.X, sc.X_last, sc.pinned, sc.params)
+
+    def _vec4(self, a):
+        a = np.ascontiguousarray(a, np.float32)
+        if a.shape != (self.N, 4):
+            raise ValueError(f"expected ({self.N}, 4) positions, got {a.shape}")
+        return a
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().oc_vgrid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params: dict):
+        p = OcVgridParams.from_dict(params)
+        check(lib().oc_vgrid_set_params(self.h, C.byref(p)))
+        self.params, self._p = dict(params), p
+
+    def set_stream(self, stream):
+        check(lib().oc_vgrid_set_stream(self.h, stream))
+
+    def get_state(self):
+        X = np.zeros((self.N, 4), np.float32)
+        XL = np.zeros((self.N, 4), np.float32)
+        check(lib().oc_vgrid_get_state(self.h, ptr(X), ptr(XL)))
+        return X, XL
+
+    def set_state(self, X=None, X_last=None):
+        Xa = None if X is None else self._vec4(X)
+        XLa = None if X_last is None else self._vec4(X_last)
+        check(lib().oc_vgrid_set_state(self.h, ptr(Xa), ptr(XLa)))
+
+    def step(self, nsteps: int = 1, vbo: int | None = None):
+        """nsteps kernel steps; vbo: device address of N float4 that receives
+        the last step's positions (the demo's mapped vertex buffer)."""
+        check(lib().oc_vgrid_step(self.h, int(nsteps), vbo))
 
 
 class PBDCloth:

@@ -170,6 +170,26 @@ struct VerletScalars {
 hipError_t launch_verlet_forces(const DevVerlet& c, const VerletScalars& s, hipStream_t st);
 hipError_t launch_verlet_step(const DevVerlet& c, const VerletScalars& s, hipStream_t st);
 
+// Grid Verlet (grid_verlet.hip): the reference's GPU Verlet kernel on a
+// width x height grid with a fixed 12-neighbour stencil.  Two position sets
+// (x, x_last) are ping-ponged: a step reads one set and writes the other.
+struct DevVgrid {
+    int nx, ny, N;
+    const float* x;   // current positions, SoA planes
+    const float* xl;  // previous positions
+    float* xo;        // next positions
+    float* xlo;       // next previous positions (the current ones, clamped)
+    const uint8_t* pin;
+    const float* w;   // w of the position buffer, carried into the vertex buffer
+    float* vbo;       // NULL or N float4 (16-byte aligned)
+};
+struct VgridScalars {
+    float mass, gx, gy, gz, damping, dt;
+    float rest[12];  // rest length of stencil position k
+    float ks[3], kd[3];
+};
+hipError_t launch_vgrid(const DevVgrid& c, const VgridScalars& s, hipStream_t st);
+
 // Position-based dynamics (pbd.hip).  Particle vectors are SoA planes;
 // constraints are stored colour by colour (list order inside a colour).
 constexpr int kPbdBlock = 256;     // particles per reduction block

@@ -1,6 +1,7 @@
-"""Minimal ctypes access to HIP events for timing work on a given stream
-(the library's handles launch on the stream set with oc_cloth_set_stream;
-the default is the null stream)."""
+"""Minimal ctypes access to the HIP runtime the library uses: events for
+timing work on a given stream (the library's handles launch on the stream
+set with oc_*_set_stream; the default is the null stream), streams, and
+device buffers to hand to entry points that take device pointers."""
 from __future__ import annotations
 
 import ctypes as C
@@ -114,3 +115,38 @@ def device_copy(nbytes: int) -> int:
         h.hipFree(a)
         h.hipFree(b)
     return nbytes
+
+
+class DeviceBuffer:
+    """nbytes of device memory from hipMalloc (for handing a device pointer,
+    such as a vertex buffer, to the library); .ptr is its address."""
+
+    def __init__(self, nbytes: int):
+        self.p = C.c_void_p()
+        self.nbytes = int(nbytes)
+        rc = hip().hipMalloc(C.byref(self.p), C.c_size_t(self.nbytes))
+        if rc != 0:
+            raise RuntimeError(f"hipMalloc failed ({rc})")
+
+    @property
+    def ptr(self) -> int:
+        return self.p.value or 0
+
+    def fill_zero(self):
+        rc = hip().hipMemset(self.p, C.c_int(0), C.c_size_t(self.nbytes))
+        if rc != 0:
+            raise RuntimeError(f"hipMemset failed ({rc})")
+
+    def to_host(self, out) -> None:
+        """Copy the whole buffer into the contiguous numpy array out
+        (synchronous hipMemcpy, device to host)."""
+        assert out.nbytes == self.nbytes and out.flags["C_CONTIGUOUS"]
+        rc = hip().hipMemcpy(C.c_void_p(out.ctypes.data), self.p, C.c_size_t(self.nbytes), C.c_int(2))
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed ({rc})")
+
+    def __del__(self):
+        try:
+            hip().hipFree(self.p)
+        except Exception:
+            pass

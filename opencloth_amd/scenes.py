@@ -372,6 +372,57 @@ def verlet_demo_scene(n: int = 21) -> Scene:
 
 
 # ---------------------------------------------------------------------------
+# grid Verlet: the GPU demos' kernel
+# (OpenCloth_Verlet_CUDA_GLSL_OPENCL_CPU/OpenCloth_Verlet_CUDA/, CUK =
+# verlet_kernel.cu, CUM = main.cpp)
+# ---------------------------------------------------------------------------
+@dataclass
+class VgridScene:
+    name: str
+    width: int
+    height: int
+    X: np.ndarray        # (N, 4) float32, w = 1
+    X_last: np.ndarray   # (N, 4) float32
+    pinned: np.ndarray   # (N,) uint8
+    params: dict         # mass, gravity, damping, dt, spacing, ks, kd
+    steps: int
+
+
+def vgrid_params(n_x: int, n_y: int, size: float = 4.0) -> dict:
+    """The demo's constants: mass 1 (CUM:100), gravity (0, -0.00981, 0)
+    (CUK:123), DEFAULT_DAMPING -0.0125 (CUM:95), dt 1/60 (CUM:102), spacing
+    (sizeX/numX, sizeY/numY) in FP32 (CUM:601), Ks 50.75 / 50.75 / 50.95 and
+    Kd -0.25 (CUK:8-10)."""
+    f32 = np.float32
+    return {"mass": 1.0, "gravity": (0.0, float(f32(-0.00981)), 0.0), "damping": float(f32(-0.0125)),
+            "dt": float(f32(1.0) / f32(60.0)),
+            "spacing": (float(f32(size) / f32(n_x - 1)), float(f32(size) / f32(n_y - 1))),
+            "ks": (float(f32(50.75)), float(f32(50.75)), float(f32(50.95))),
+            "kd": (-0.25, -0.25, -0.25)}
+
+
+def vgrid_demo_scene(n: int = 64) -> VgridScene:
+    """The GPU demo's cloth (CUM:56, CUM:436-444): n x n texels (numX = n-1),
+    vertex j*n + i at (((i/(n-1))*2-1)*2, 5, (j/(n-1))*4, 1), at rest, pins
+    0 and n-1 (CUK:126)."""
+    X3 = grid_positions(n, n, 4.0, 5.0)
+    X = np.concatenate([X3, np.ones((n * n, 1), np.float32)], axis=1)
+    pinned = np.zeros(n * n, np.uint8)
+    pinned[0] = pinned[n - 1] = 1
+    return VgridScene(f"vgrid_demo_{n}x{n}", n, n, X, X.copy(), pinned, vgrid_params(n, n), 600)
+
+
+def vgrid_large_scene(n: int = 512, seed: int = 0) -> VgridScene:
+    """The demo's cloth and constants at n x n, positions perturbed by
+    N(0, 1e-3*spacing) (x_last = the perturbed positions)."""
+    sc = vgrid_demo_scene(n)
+    spacing = 4.0 / (n - 1)
+    X = sc.X.copy()
+    X[:, :3] = (X[:, :3] + (1e-3 * spacing) * normal(seed, 3 * n * n).reshape(-1, 3)).astype(np.float32)
+    return VgridScene(f"vgrid_{n}x{n}", n, n, X, X.copy(), sc.pinned, sc.params, 100)
+
+
+# ---------------------------------------------------------------------------
 # position-based dynamics (OpenCloth_PositionBasedDynamics/.../main.cpp, PBD)
 # ---------------------------------------------------------------------------
 # AddDistanceConstraint(a, b, k) (PBD:148-162) and the triangle form of

@@ -100,6 +100,7 @@ class Oracle:
         L.orc_pbd_project.argtypes = [_P, _P, _P, C.c_int]
         L.orc_pbd_integrate.argtypes = [_P, _P, _P, _P, _P]
         L.orc_pbd_step.argtypes = [_P, _P, _P, _P, C.c_int, C.c_int, _P, _P, _P, C.c_float]
+        L.orc_vgrid_step.argtypes = [C.c_int, C.c_int, _P, _P, _P, _P, C.c_int]
 
     def set_threads(self, n: int) -> int:
         """Worker threads of the implicit step's parallel loops (results are
@@ -131,6 +132,20 @@ class Oracle:
 
     def pbd(self, X0, pinned, mass, distance, bending, solver_iterations) -> "PbdSystem":
         return PbdSystem(self, X0, pinned, mass, distance, bending, solver_iterations)
+
+    def vgrid_rollout(self, width, height, X4, XL4, pinned, prm, nsteps):
+        """nsteps of the GPU demos' Verlet kernel (orc_vgrid_step) on copies of
+        the (N, 4) position arrays; prm: dict with mass, gravity, damping, dt,
+        spacing, ks, kd.  Returns (X4, XL4) in this oracle's dtype."""
+        X = self.arr(X4).copy()
+        XL = self.arr(XL4).copy()
+        pin = np.ascontiguousarray(pinned, np.uint8)
+        assert X.shape == XL.shape == (width * height, 4) and pin.shape == (width * height,)
+        p = np.array([prm["mass"], *prm["gravity"], prm["damping"], prm["dt"], *prm["spacing"],
+                      *prm["ks"], *prm["kd"]], np.float64)
+        self.L.orc_vgrid_step(int(width), int(height), _ptr(X), _ptr(XL), _ptr(pin), _ptr(p),
+                              int(nsteps))
+        return X, XL
 
 
 def make_params(p, nx=0, ny=0, tx=0, ty=0, dot_mode=0, **over) -> Params:

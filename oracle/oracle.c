@@ -1237,3 +1237,94 @@ EXPORT void orc_verlet_step(const orc_sys* s, real* X, real* XL, double dt, doub
     if (M) orc_ellipsoid(s, X, XL, M, Minv, center, radius, 1);
     free(F);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Grid Verlet: the GPU demos' kernel                                        */
+/* ------------------------------------------------------------------------ */
+/* CUK = OpenCloth_Verlet_CUDA_GLSL_OPENCL_CPU/OpenCloth_Verlet_CUDA/verlet_kernel.cu.
+ * The kernel includes cutil_math.h, a helper header of the CUDA SDK of its
+ * time (GPU Computing SDK 3.x) that is not part of the reference.  It
+ * defines the float2/float3 operators the kernel uses: componentwise + - *,
+ * dot as a left-to-right sum of products, length = sqrtf(dot(v, v)) and
+ * normalize = v * rsqrtf(dot(v, v)).  rsqrtf is an approximate reciprocal
+ * square root and nvcc contracts a*b+c into FMAs by default, so the CUDA
+ * kernel's bits are not reproducible; this restatement (and the HIP kernel)
+ * uses correctly rounded operations in the kernel's expression order, with
+ * normalize = v * (1/sqrt(dot(v, v))) as in spring_terms and scalar
+ * division componentwise.
+ *
+ * Stencil position k = 0..11 of getNextNeighbor (CUK:14-64), (column, row)
+ * offsets; k < 4 structural, k < 8 shear, else bend. */
+static const int vg_dx[12] = {1, 0, -1, 0, 1, -1, -1, 1, 2, 0, -2, 0};
+static const int vg_dy[12] = {0, -1, 0, 1, -1, -1, 1, 1, 0, -2, 0, 2};
+
+/* nsteps of the kernel's step (CUK:119-195) on a width x height grid of
+ * float4 positions X4 and previous positions XL4 (AoS, w carried):
+ *   v = (x - x_last)/dt,  F = gravity*mass + v*damping            CUK:121-124
+ *   F += spring force of each on-grid stencil neighbour, rest
+ *        length |(dx*sx, dy*sy)|, the class's ks and kd          CUK:130-176
+ *   a = F/mass, 0 on pinned vertices (mass zeroed)               CUK:126-127, 178-180
+ *   y < 0 of the current position clamped to 0                   CUK:183-184
+ *   x' = (2x - x_last) + (a dt) dt,  x_last' = x (clamped)       CUK:187-189
+ * prm = {mass, gx, gy, gz, damping, dt, sx, sy, ks[3], kd[3]}.
+ * Pinned by: tests/test_oracle_vgrid.py -- free fall of an unpinned grid at
+ * rest spacing against the closed form x_n = x_0 + g dt^2 n(n+1)/2 (springs
+ * at rest length exert nothing, so a wrong rest length, class or offset
+ * fails), an independent scatter over the enumerated springs (FP64), zero
+ * total force with no gravity, damping or pins, and the clamp. */
+EXPORT void orc_vgrid_step(int width, int height, real* X4, real* XL4, const uint8_t* pin,
+                           const double* prm, int nsteps) {
+    const int N = width * height;
+    const real m = (real)prm[0], h = (real)prm[5], damp = (real)prm[4];
+    const real g[3] = {(real)prm[1], (real)prm[2], (real)prm[3]};
+    const real sx = (real)prm[6], sy = (real)prm[7];
+    real rest[12], ks[3], kd[3];
+    for (int k = 0; k < 12; k++) {
+        real a = (real)vg_dx[k] * sx, b = (real)vg_dy[k] * sy;
+        rest[k] = SQRT_(a * a + b * b);
+    }
+    for (int c = 0; c < 3; c++) {
+        ks[c] = (real)prm[8 + c];
+        kd[c] = (real)prm[11 + c];
+    }
+    real* V = (real*)malloc(sizeof(real) * 3 * (N > 0 ? N : 1));
+    real* Xn = (real*)malloc(sizeof(real) * 4 * (N > 0 ? N : 1));
+    real* XLn = (real*)malloc(sizeof(real) * 4 * (N > 0 ? N : 1));
+    for (int step = 0; step < nsteps; step++) {
+        OMP_FOR
+        for (int i = 0; i < N; i++)
+            for (int c = 0; c < 3; c++) V[3 * i + c] = (X4[4 * i + c] - XL4[4 * i + c]) / h;
+        OMP_FOR
+        for (int i = 0; i < N; i++) {
+            const int ix = i % width, iy = i / width;
+            real x[3] = {X4[4 * i], X4[4 * i + 1], X4[4 * i + 2]};
+            real F[3];
+            for (int c = 0; c < 3; c++) F[c] = g[c] * m + V[3 * i + c] * damp;
+            for (int k = 0; k < 12; k++) {
+                const int jx = ix + vg_dx[k], jy = iy + vg_dy[k];
+                if (jy < 0 || jy > height - 1 || jx < 0 || jx > width - 1) continue;
+                const int j = jy * width + jx;
+                real xj[3] = {X4[4 * j], X4[4 * j + 1], X4[4 * j + 2]};
+                real f[3];
+                spring_terms(x, xj, V + 3 * i, V + 3 * j, rest[k], ks[k >> 2], kd[k >> 2], f, NULL,
+                             NULL);
+                for (int c = 0; c < 3; c++) F[c] += f[c];
+            }
+            real a[3] = {0, 0, 0};
+            if (!pin[i])
+                for (int c = 0; c < 3; c++) a[c] = F[c] / m;
+            if (x[1] < 0) x[1] = 0;
+            for (int c = 0; c < 3; c++) {
+                Xn[4 * i + c] = (x[c] * (real)2 - XL4[4 * i + c]) + (a[c] * h) * h;
+                XLn[4 * i + c] = x[c];
+            }
+            Xn[4 * i + 3] = X4[4 * i + 3];
+            XLn[4 * i + 3] = XL4[4 * i + 3];
+        }
+        memcpy(X4, Xn, sizeof(real) * 4 * N);
+        memcpy(XL4, XLn, sizeof(real) * 4 * N);
+    }
+    free(V);
+    free(Xn);
+    free(XLn);
+}
