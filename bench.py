@@ -83,10 +83,14 @@ def bench_entries(steps: int = 100) -> dict:
               masses read, indices, rest length and k' read, two particles
               written; bending likewise with three; prediction, damping sums
               and integration 132N)
+      verlet_grid  the GPU demos' Verlet kernel (12-neighbour stencil, rest
+              lengths from the spacing) on a 512x512 grid with the demo's
+              constants, one launch per step; bytes 49N (x and x_last read,
+              both written, one pin byte)
       implicit_step_physics  the implicit demo's full StepPhysics (21x21,
               step + collider + Provot), launch-bound."""
     from opencloth_amd import scenes
-    from opencloth_amd.cloth import ImplicitCloth, PBDCloth, VerletCloth
+    from opencloth_amd.cloth import ImplicitCloth, PBDCloth, VerletCloth, VerletGrid
     from opencloth_amd.hiptime import Event
 
     def timed(run, n):
@@ -128,6 +132,15 @@ def bench_entries(steps: int = 100) -> dict:
                   "launches_per_step": 5 + ns * (nd_col + nb_col + int(sc.params.ground)) + 1,
                   "algorithmic_bytes": b, "roofline_frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     pc.close()
+
+    sc = scenes.vgrid_large_scene(n)
+    vg = VerletGrid.from_scene(sc)
+    ms = timed(lambda k: vg.step(k), steps)
+    b = 49 * N
+    out["verlet_grid"] = {"workload": f"{n}x{n} grid Verlet step (12-neighbour stencil)",
+                          "ms_per_step": ms, "vertex_steps_per_s": N / (ms * 1e-3),
+                          "algorithmic_bytes": b, "roofline_frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    vg.close()
 
     sc = scenes.demo_scene()
     cl = ImplicitCloth.from_scene(sc)

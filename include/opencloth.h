@@ -264,6 +264,9 @@ int oc_vgrid_get_state(oc_vgrid grid, float* x, float* x_last);
  * the mapped vertex buffer of VerletCUDA (CUH:81-100, CUM:597-607).  Stream
  * ordered. */
 int oc_vgrid_step(oc_vgrid grid, int32_t nsteps, float* vbo);
+/* Waits for the grid's stream (VerletCUDA blocks until its kernel is done,
+ * CUH:96). */
+int oc_vgrid_synchronize(oc_vgrid grid);
 
 /* ---- position-based dynamics (PBD) --------------------------------------- */
 /* PBD = OpenCloth_PositionBasedDynamics/OpenCloth_PositionBasedDynamics/main.cpp,

@@ -24,6 +24,7 @@
  *   IE  = OpenCloth_ImplicitEuler/OpenCloth_ImplicitEuler/main.cpp
  *   VL  = OpenCloth_Verlet/OpenCloth_Verlet/main.cpp
  *   PBD = OpenCloth_PositionBasedDynamics/OpenCloth_PositionBasedDynamics/main.cpp
+ *   CUH = OpenCloth_Verlet_CUDA_GLSL_OPENCL_CPU/OpenCloth_Verlet_CUDA/verlet.cu
  */
 #ifndef OPENCLOTH_HPP
 #define OPENCLOTH_HPP
@@ -51,6 +52,12 @@ inline void check(int rc, const char* call) {
     if (rc != OC_OK) throw Error(rc, std::string(call) + ": " + oc_last_error());
 }
 
+template <class Vec4>
+inline float* floats4(std::vector<Vec4>& v) {
+    static_assert(sizeof(Vec4) == 4 * sizeof(float) && std::is_standard_layout<Vec4>::value,
+                  "Vec4 must be four packed floats, like glm::vec4");
+    return reinterpret_cast<float*>(v.data());
+}
 template <class Vec3>
 inline float* floats(std::vector<Vec3>& v) {
     static_assert(sizeof(Vec3) == 3 * sizeof(float) && std::is_standard_layout<Vec3>::value,
@@ -246,6 +253,46 @@ class PositionBasedDynamics {
     std::vector<Vec3>& X_;
     std::vector<Vec3>& V_;
     oc_pbd h_ = nullptr;
+};
+
+/* The GPU Verlet demo's device state (CUH): the constructor does InitCUDA
+ * and the first UploadCUDA (CUH:16-25, CUH:53-79), Verlet() is VerletCUDA
+ * (CUH:81-100) and the destructor ShutdownCUDA (CUH:27-45).  Vec4 is the
+ * caller's four-float vector (glm::vec4 in the demo).  As in the demo, the
+ * positions stay on the device between steps and reach the host only
+ * through the vertex buffer; Download() copies X and X_last back. */
+template <class Vec4>
+class GridVerlet {
+   public:
+    GridVerlet(int texsize_x, int texsize_y, std::vector<Vec4>& X, std::vector<Vec4>& X_last,
+               const oc_vgrid_params& params, const uint8_t* pinned = nullptr)
+        : X_(X), XL_(X_last) {
+        const size_t n = (size_t)texsize_x * (size_t)texsize_y;
+        need(X.size() == n && X_last.size() == n, "GridVerlet: X and X_last need texsize_x*texsize_y entries");
+        check(oc_vgrid_create(texsize_x, texsize_y, floats4(X), floats4(X_last), pinned, &params, &h_),
+              "oc_vgrid_create");
+    }
+    ~GridVerlet() { oc_vgrid_destroy(h_); }
+    GridVerlet(const GridVerlet&) = delete;
+    GridVerlet& operator=(const GridVerlet&) = delete;
+
+    /* VerletCUDA: one step; pos_vbo is NULL or a device array of float4 that
+     * receives the new positions.  Returns when the step is done. */
+    void Verlet(float* pos_vbo = nullptr) {
+        check(oc_vgrid_step(h_, 1, pos_vbo), "oc_vgrid_step");
+        check(oc_vgrid_synchronize(h_), "oc_vgrid_synchronize");
+    }
+    /* n steps, stream-ordered (no wait). */
+    void Step(int n, float* pos_vbo = nullptr) { check(oc_vgrid_step(h_, n, pos_vbo), "oc_vgrid_step"); }
+    void SetParams(const oc_vgrid_params& p) { check(oc_vgrid_set_params(h_, &p), "oc_vgrid_set_params"); }
+    void Upload() { check(oc_vgrid_set_state(h_, floats4(X_), floats4(XL_)), "oc_vgrid_set_state"); }
+    void Download() { check(oc_vgrid_get_state(h_, floats4(X_), floats4(XL_)), "oc_vgrid_get_state"); }
+    oc_vgrid handle() const { return h_; }
+
+   private:
+    std::vector<Vec4>& X_;
+    std::vector<Vec4>& XL_;
+    oc_vgrid h_ = nullptr;
 };
 
 }  // namespace opencloth

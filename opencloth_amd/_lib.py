@@ -74,6 +74,7 @@ SIGNATURES = {
     "oc_vgrid_set_state": (C.c_int, [_P, _P, _P]),
     "oc_vgrid_get_state": (C.c_int, [_P, _P, _P]),
     "oc_vgrid_step": (C.c_int, [_P, _I, _P]),
+    "oc_vgrid_synchronize": (C.c_int, [_P]),
     "oc_pbd_create": (C.c_int, [_I, _P, _P, _P, _I, _P, _I, _P, _P]),
     "oc_pbd_destroy": (C.c_int, [_P]),
     "oc_pbd_set_stream": (C.c_int, [_P, _P]),

@@ -346,6 +346,9 @@ class VerletGrid:
         the last step's positions (the demo's mapped vertex buffer)."""
         check(lib().oc_vgrid_step(self.h, int(nsteps), vbo))
 
+    def synchronize(self):
+        check(lib().oc_vgrid_synchronize(self.h))
+
 
 class PBDCloth:
     """Position-based-dynamics cloth (oc_pbd), the PBD demo's step

@@ -230,4 +230,10 @@ int oc_vgrid_step(oc_vgrid g, int32_t nsteps, float* vbo) {
     return OC_OK;
 }
 
+int oc_vgrid_synchronize(oc_vgrid g) {
+    if (!g) return set_error(OC_ERR_INVALID_ARGUMENT, "grid is NULL");
+    OC_HIP(hipStreamSynchronize(g->stream));
+    return OC_OK;
+}
+
 }  // extern "C"

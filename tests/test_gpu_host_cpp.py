@@ -1,4 +1,4 @@
-"""The C++ demo programs (opencloth_amd/host/{ie,vl,pbd}_demo.cpp: each
+"""The C++ demo programs (opencloth_amd/host/{ie,vl,pbd,vg}_demo.cpp: each
 reference demo's scene and StepPhysics loop through include/opencloth.hpp
 over the C ABI) against the FP32 oracle on the same scene, bit for bit, after
 enough steps for the cloth to reach the collider."""
@@ -84,3 +84,19 @@ def test_cpp_pbd_demo_matches_oracle(tmp_path):
     assert X[:, 1].min() < 2.0  # draped on the collider
     assert np.array_equal(Xg.view(np.uint32), X.view(np.uint32))
     assert np.array_equal(Vg.view(np.uint32), V.view(np.uint32))
+
+
+def test_cpp_grid_verlet_demo_matches_oracle(tmp_path):
+    from oracle import Oracle
+
+    steps = 600
+    b = run_demo("vg_demo", steps, tmp_path)
+    n, u, v = struct.unpack("<iii", b[:12])
+    Xg = np.frombuffer(b, np.float32, 4 * n, 12).reshape(n, 4)
+    XLg = np.frombuffer(b, np.float32, 4 * n, 12 + 16 * n).reshape(n, 4)
+    sc = scenes.vgrid_demo_scene(64)
+    assert (n, u, v) == (sc.width * sc.height, sc.width, sc.height)
+    Xr, XLr = Oracle(32).vgrid_rollout(u, v, sc.X, sc.X_last, sc.pinned, sc.params, steps)
+    assert np.abs(Xr[:, :3] - sc.X[:, :3]).max() > 0.3  # the free edge fell
+    assert np.array_equal(Xg.view(np.uint32), Xr.view(np.uint32))
+    assert np.array_equal(XLg.view(np.uint32), XLr.view(np.uint32))

@@ -1,5 +1,5 @@
 """The C++ host layer (include/opencloth.hpp) and the demos built on it
-(opencloth_amd/host/{ie,vl,pbd}_demo.cpp): each program's scene, made by its
+(opencloth_amd/host/{ie,vl,pbd,vg}_demo.cpp): each program's scene, made by its
 own C++ restatement of the reference demo's InitGL, equals the Python demo
 scene bit for bit, so the GPU tests of their StepPhysics loops run the
 oracle on the same inputs (CPU only: --scene-only makes no GPU call)."""
@@ -55,6 +55,25 @@ def test_verlet_demo_scene_matches(tmp_path):
     assert np.array_equal(ell[16:32].view(np.uint32), e["Minv"].view(np.uint32))
 
 
+def test_grid_verlet_demo_scene_matches(tmp_path):
+    out = tmp_path / "scene.bin"
+    r = subprocess.run([host_program("vg_demo"), "--scene-only", "--out", str(out)],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    b = open(out, "rb").read()
+    n, u, v = struct.unpack("<iii", b[:12])
+    X = np.frombuffer(b, np.float32, 4 * n, 12).reshape(n, 4)
+    prm = np.frombuffer(b, np.float32, 14, 12 + 16 * n)
+    assert 12 + 16 * n + 56 == len(b)
+    sc = scenes.vgrid_demo_scene(64)
+    assert (n, u, v) == (sc.width * sc.height, sc.width, sc.height)
+    assert np.array_equal(X.view(np.uint32), sc.X.view(np.uint32))
+    p = sc.params
+    expect = np.array([p["mass"], *p["gravity"], p["damping"], p["dt"], *p["spacing"], *p["ks"], *p["kd"]],
+                      np.float32)
+    assert np.array_equal(prm.view(np.uint32), expect.view(np.uint32))
+
+
 def test_pbd_demo_scene_matches(tmp_path):
     out = tmp_path / "scene.bin"
     r = subprocess.run([host_program("pbd_demo"), "--scene-only", "--out", str(out)],
@@ -83,7 +102,7 @@ def test_demo_rejects_bad_arguments(tmp_path):
 
 
 def test_every_class_instantiates(tmp_path):
-    """All three classes and every member compile when instantiated and
+    """All four classes and every member compile when instantiated and
     link against the library (host compile and link only; nothing runs)."""
     import os
 
@@ -96,6 +115,8 @@ struct v3 { float x, y, z; };
 template class opencloth::ImplicitEuler<v3>;
 template class opencloth::Verlet<v3>;
 template class opencloth::PositionBasedDynamics<v3>;
+struct v4 { float x, y, z, w; };
+template class opencloth::GridVerlet<v4>;
 int main(int argc, char**) {
     if (argc > 1) {  // never taken: only checks that every call links
         std::vector<v3> X(4), V(4), F;
@@ -114,6 +135,10 @@ int main(int argc, char**) {
         std::vector<oc_pbd_distance> d; std::vector<oc_pbd_bending> bd;
         opencloth::PositionBasedDynamics<v3> c(X, V, pin, d, bd, pp);
         c.StepPhysics(0.01f, &e); c.Upload();
+        std::vector<v4> X4(4), XL4(4);
+        oc_vgrid_params gp = {};
+        opencloth::GridVerlet<v4> g(2, 2, X4, XL4, gp);
+        g.Verlet(nullptr); g.Step(3); g.SetParams(gp); g.Upload(); g.Download();
     }
     return 0;
 }
