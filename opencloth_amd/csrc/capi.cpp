@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "capi_common.hpp"
 #include "oc_internal.hpp"
 
 namespace {
@@ -21,19 +22,11 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
-#define OC_HIP(call)                                                                     \
-    do {                                                                                 \
-        hipError_t e_ = (call);                                                          \
-        if (e_ != hipSuccess)                                                            \
-            return fail(OC_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+using oc::align_up;
 
 }  // namespace
 
 namespace oc {
-// Records the message returned by oc_last_error on this thread.
 int set_error(int code, const std::string& msg) { return fail(code, msg); }
 }  // namespace oc
 

@@ -13,24 +13,13 @@
 #include <string>
 #include <vector>
 
+#include "capi_common.hpp"
 #include "oc_internal.hpp"
-
-namespace oc {
-int set_error(int code, const std::string& msg);  // capi.cpp
-}
 
 namespace {
 
+using oc::align_up;
 using oc::set_error;
-
-#define OC_HIP(call)                                                                          \
-    do {                                                                                      \
-        hipError_t e_ = (call);                                                               \
-        if (e_ != hipSuccess)                                                                 \
-            return set_error(OC_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
 }  // namespace
 
