@@ -85,3 +85,24 @@ def test_tile_shape_is_reported_without_device():
 
     tx, ty = tile_shape()
     assert tx > 0 and ty > 0 and (tx * ty) % 64 == 0 and tx * ty <= 1024
+
+
+def test_vgrid_create_rejects_bad_input_without_device():
+    """oc_vgrid_create checks the grid, the positions and the parameters
+    before touching the device; NULL handles come back as errors."""
+    import numpy as np
+
+    from opencloth_amd import scenes
+    from opencloth_amd._lib import OC_ERR_INVALID_ARGUMENT, OcVgridParams, lib, ptr
+
+    X = np.zeros((4, 4), np.float32)
+    good = scenes.vgrid_params(2, 2)
+    h = ctypes.c_void_p()
+    for w, hgt, x, change in ((0, 4, X, {}), (2, 2, None, {}), (2, 2, X, {"dt": 0.0}),
+                              (2, 2, X, {"mass": -1.0}), (1 << 15, 1 << 14, X, {})):
+        p = OcVgridParams.from_dict(dict(good, **change))
+        rc = lib().oc_vgrid_create(w, hgt, ptr(x), None, None, ctypes.byref(p), ctypes.byref(h))
+        assert rc == OC_ERR_INVALID_ARGUMENT and not h.value, (w, hgt, change)
+        assert lib().oc_last_error() != b""
+    assert lib().oc_vgrid_step(None, 1, None) == OC_ERR_INVALID_ARGUMENT
+    assert lib().oc_vgrid_destroy(None) == 0
