@@ -98,3 +98,25 @@ def test_params_between_steps_and_round_trip(o32):
     g.set_state(X=X2)
     Xg, XLg = g.get_state()
     assert np.array_equal(Xg, X2) and np.array_equal(XLg, XL)
+
+
+def test_caller_stream_and_rest_start(o32):
+    """Work ordered on a caller's HIP stream gives the same bits, and a NULL
+    x_last starts at rest (x_last = x)."""
+    from opencloth_amd.cloth import VerletGrid
+    from opencloth_amd.hiptime import Stream
+
+    sc = scenes.vgrid_large_scene(70, seed=9)
+    ref = VerletGrid(sc.width, sc.height, sc.X, sc.X, sc.pinned, sc.params)
+    ref.step(7)
+    g = VerletGrid(sc.width, sc.height, sc.X, None, sc.pinned, sc.params)
+    stream = Stream()
+    g.set_stream(stream.handle)
+    g.step(7)
+    g.synchronize()
+    Xg, XLg = g.get_state()
+    g.set_stream(None)
+    Xr, XLr = ref.get_state()
+    assert np.array_equal(bits(Xg), bits(Xr)) and np.array_equal(bits(XLg), bits(XLr))
+    Xo, XLo = o32.vgrid_rollout(sc.width, sc.height, sc.X, sc.X, sc.pinned, sc.params, 7)
+    assert np.array_equal(bits(Xg), bits(Xo)) and np.array_equal(bits(XLg), bits(XLo))
