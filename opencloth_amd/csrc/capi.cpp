@@ -8,6 +8,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "capi_common.hpp"
@@ -40,6 +41,8 @@ struct oc_cloth_s {
     void* scratch = nullptr;      // CG partials and scalars
     int* ctl_host = nullptr;      // pinned copy of ctl
     oc::DevCloth dc{};
+    float* alt_x = nullptr;  // the other (x, v) set: a fused update writes here,
+    float* alt_v = nullptr;  // then the two sets swap
     bool assembled = false;
     // differentiable rollouts: checkpoints, one segment of states, adjoint vectors
     void* adj_arena = nullptr;
@@ -232,7 +235,7 @@ int oc_cloth_create(int32_t nx, int32_t ny, const float* x0, const float* v0,
     const size_t pair_k = align_up(sizeof(uint16_t) * std::max<size_t>(t.pair_mask.size(), 1));
     const size_t sp = align_up(sizeof(float) * S);
     const size_t pin = align_up(N);
-    const size_t total = 11 * vec + blocks + ainc_s + ainc_m + pair_m + pair_k + 3 * sp + pin;
+    const size_t total = 13 * vec + blocks + ainc_s + ainc_m + pair_m + pair_k + 3 * sp + pin;
     if (hipMalloc(&c->arena, total) != hipSuccess) {
         delete c;
         return fail(OC_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(total) + " bytes failed");
@@ -265,6 +268,8 @@ int oc_cloth_create(int32_t nx, int32_t ny, const float* x0, const float* v0,
     d.ks = (const float*)take(sp);
     d.kd = (const float*)take(sp);
     d.pin = (const uint8_t*)take(pin);
+    c->alt_x = (float*)take(vec);  // last, so the other arrays keep their offsets
+    c->alt_v = (float*)take(vec);
 
     auto cleanup = [&](int code) {
         (void)hipFree(c->arena);
@@ -420,11 +425,23 @@ int oc_cloth_step(oc_cloth c, int32_t nsteps) {
     if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
     if (nsteps < 0) return fail(OC_ERR_INVALID_ARGUMENT, "nsteps < 0");
     const oc::StepScalars s = scalars(c, c->prm.dt);
+    // Steps after the first apply the previous step's update inside their
+    // assembly (one launch fewer per step, the same bits); the last step's
+    // update, which also records the solve's status, is a launch of its own.
     for (int k = 0; k < nsteps; k++) {
-        OC_HIP(oc::launch_assemble(c->dc, s, c->stream));
+        if (k == 0) {
+            OC_HIP(oc::launch_assemble(c->dc, s, c->stream));
+        } else {
+            oc::DevCloth d = c->dc;
+            d.xo = c->alt_x;
+            d.vo = c->alt_v;
+            OC_HIP(oc::launch_assemble(d, s, c->stream, true));
+            std::swap(c->dc.x, c->alt_x);
+            std::swap(c->dc.v, c->alt_v);
+        }
         OC_HIP(oc::launch_cg(c->dc, s, c->stream, false));
-        OC_HIP(oc::launch_update(c->dc, s, c->stream, true));
     }
+    if (nsteps > 0) OC_HIP(oc::launch_update(c->dc, s, c->stream, true));
     c->assembled = false;
     return OC_OK;
 }

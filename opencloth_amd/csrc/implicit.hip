@@ -107,7 +107,10 @@ struct AssembleShared {
     float ws[TX * TY / 64];
 };
 
-template <int TX, int TY, int WA>
+// UPD: the window's x and v are the previous step's state before its
+// update; apply the update (k_update's arithmetic) on the fly and write the
+// tile's updated x, v to xo, vo (other blocks still read x and v).
+template <int TX, int TY, int WA, bool UPD>
 __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) {
     constexpr int NT = TX * TY;
     constexpr int WX = TX + 2 * H, WY = TY + 2 * H;
@@ -148,7 +151,15 @@ __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) 
                 const int g = gy * c.nx + gx;
                 xw = ld3o(c.x, N, g);
                 vw = ld3o(c.v, N, g);
-                if (s.warm) dw = ld3o(c.dv, N, g);
+                if (UPD) {
+                    const V3 dp = ld3o(c.dv, N, g);
+                    vw = {vw.x + dp.x, vw.y + dp.y, vw.z + dp.z};
+                    xw = {xw.x + s.h * vw.x, xw.y + s.h * vw.y, xw.z + s.h * vw.z};
+                    if ((s.flags & OC_FLAG_GROUND_CLAMP) && xw.y < 0.0f) xw.y = 0.0f;
+                    if (s.warm) dw = dp;
+                } else if (s.warm) {
+                    dw = ld3o(c.dv, N, g);
+                }
             }
             sh.wx[0][wy][wx] = xw.x;
             sh.wx[1][wy][wx] = xw.y;
@@ -172,6 +183,10 @@ __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) 
         const V3 vi = {sh.wv[0][cy][cx], sh.wv[1][cy][cx], sh.wv[2][cy][cx]};
         const V3 dvi = {sh.wdv[0][cy][cx], sh.wdv[1][cy][cx], sh.wdv[2][cy][cx]};
         if (!s.warm) st3o(c.dv, N, i, dvi);
+        if (UPD) {
+            st3o(c.xo, N, i, xi);
+            st3o(c.vo, N, i, vi);
+        }
         V3 acc = {0.0f, 0.0f, 0.0f}, kv = {0.0f, 0.0f, 0.0f};
         if (!pinned) acc = {0.0f + s.mass * s.gx, 0.0f + s.mass * s.gy, 0.0f + s.mass * s.gz};
         V3 adv = {s.mass * dvi.x, s.mass * dvi.y, s.mass * dvi.z};
@@ -710,13 +725,21 @@ hipError_t launch_forces(const DevCloth& c, const StepScalars& s, hipStream_t st
     return hipGetLastError();
 }
 
-hipError_t launch_assemble(const DevCloth& c, const StepScalars& s, hipStream_t st) {
-    if (c.wa <= 16)
-        hipLaunchKernelGGL((k_assemble<kTileX, kTileY, 16>), dim3(c.nt), dim3(kTileX * kTileY), 0,
-                           st, c, s);
-    else
-        hipLaunchKernelGGL((k_assemble<kTileX, kTileY, kMaxAssemblyWidth>), dim3(c.nt),
-                           dim3(kTileX * kTileY), 0, st, c, s);
+hipError_t launch_assemble(const DevCloth& c, const StepScalars& s, hipStream_t st, bool fused_update) {
+    const dim3 grid(c.nt), block(kTileX * kTileY);
+    if (c.wa <= 16) {
+        if (fused_update)
+            hipLaunchKernelGGL((k_assemble<kTileX, kTileY, 16, true>), grid, block, 0, st, c, s);
+        else
+            hipLaunchKernelGGL((k_assemble<kTileX, kTileY, 16, false>), grid, block, 0, st, c, s);
+    } else {
+        if (fused_update)
+            hipLaunchKernelGGL((k_assemble<kTileX, kTileY, kMaxAssemblyWidth, true>), grid, block, 0, st,
+                               c, s);
+        else
+            hipLaunchKernelGGL((k_assemble<kTileX, kTileY, kMaxAssemblyWidth, false>), grid, block, 0,
+                               st, c, s);
+    }
     return hipGetLastError();
 }
 

@@ -108,6 +108,10 @@ struct DevCloth {
                       // [4] grid-barrier time-out, [6] grid-barrier counter
     int persistent;   // > 0: run all CG iterations in one cooperative launch of
                       // this many blocks (each serving several tiles)
+    // fused update (assembly only): x + h(v + dv) and v + dv are written
+    // here; last, so the fields the other kernels read keep their offsets
+    float* xo;
+    float* vo;
 };
 
 // Blocks of the persistent CG launch (all resident at once), 0 if unsupported.
@@ -120,7 +124,12 @@ struct StepScalars {
 
 // Kernel launchers (implicit.hip).  All return hipError_t of the launch.
 hipError_t launch_forces(const DevCloth& c, const StepScalars& s, hipStream_t st);
-hipError_t launch_assemble(const DevCloth& c, const StepScalars& s, hipStream_t st);
+// fused_update: first apply the previous step's update (v += dv, x += h v,
+// ground clamp) to x and v, writing the result to xo and vo, then assemble
+// at the updated state (the same bits as an update launch followed by the
+// assembly).
+hipError_t launch_assemble(const DevCloth& c, const StepScalars& s, hipStream_t st,
+                           bool fused_update = false);
 // finish = false leaves the iteration count and status to launch_update(...,
 // finish = true), the step's last launch.
 hipError_t launch_cg(const DevCloth& c, const StepScalars& s, hipStream_t st, bool finish = true);
