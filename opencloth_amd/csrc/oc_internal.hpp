@@ -79,37 +79,39 @@ int build_grid_topology(int nx, int ny, const oc_spring* springs, int n_springs,
 struct DevCloth {
     int nx, ny, N, ntx, nty, nt;
     int wa, wp;
-    float* x;
-    float* v;
+    // what the CG launches read, first and together (a launch's scalar
+    // loads of its arguments then touch few cache lines)
     float* dv;
     float* r;
     float* d0;
     float* d1;
     float* q;
-    float* b;
-    float* f;
     float* s;       // z = P^-1 r of the Jacobi-preconditioned solve
     float* pinv;    // Jacobi preconditioner 1/diag(A)
     const uint8_t* pin;
     float* blocks;  // n_planes*6*N
-    const int32_t* ainc_s;
-    const uint16_t* ainc_m;
     const uint16_t* pair_m;
     const uint16_t* pair_mask;  // stencil form: bit k = pair at stencil position k
-    int stencil;                // nonzero: pairs follow the kStencil order
-    int stencil_plane[kStencil];
-    const float* L0;
-    const float* ks;
-    const float* kd;
     float* del_part;  // (max_it+1)*nt
     float* sig_part;  // max_it*nt
     float* scal;      // max_it+1 reduced r.r values
     int* ctl;         // [0] done, [1] iterations, [2] status bits, [3] sticky status,
                       // [4] grid-barrier time-out, [6] grid-barrier counter
+    int stencil;      // nonzero: pairs follow the kStencil order
+    int stencil_plane[kStencil];
     int persistent;   // > 0: run all CG iterations in one cooperative launch of
                       // this many blocks (each serving several tiles)
-    // fused update (assembly only): x + h(v + dv) and v + dv are written
-    // here; last, so the fields the other kernels read keep their offsets
+    // the assembly's and the other kernels' fields
+    float* x;
+    float* v;
+    float* b;
+    float* f;
+    const int32_t* ainc_s;
+    const uint16_t* ainc_m;
+    const float* L0;
+    const float* ks;
+    const float* kd;
+    // fused update (assembly only): x + h(v + dv) and v + dv are written here
     float* xo;
     float* vo;
 };
