@@ -425,10 +425,19 @@ int oc_cloth_step(oc_cloth c, int32_t nsteps) {
     if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
     if (nsteps < 0) return fail(OC_ERR_INVALID_ARGUMENT, "nsteps < 0");
     const oc::StepScalars s = scalars(c, c->prm.dt);
-    // Steps after the first apply the previous step's update inside their
-    // assembly (one launch fewer per step, the same bits); the last step's
-    // update, which also records the solve's status, is a launch of its own.
+    // With a warm CG start, steps after the first apply the previous step's
+    // update inside their assembly (one launch fewer per step, the same
+    // bits); the last step's update, which also records the solve's status,
+    // is a launch of its own.  A cold start is not fused: its assembly zeroes
+    // dv, which neighbouring blocks of a fused assembly would still be
+    // reading for the update.
     for (int k = 0; k < nsteps; k++) {
+        if (!s.warm) {
+            OC_HIP(oc::launch_assemble(c->dc, s, c->stream));
+            OC_HIP(oc::launch_cg(c->dc, s, c->stream, false));
+            OC_HIP(oc::launch_update(c->dc, s, c->stream, true));
+            continue;
+        }
         if (k == 0) {
             OC_HIP(oc::launch_assemble(c->dc, s, c->stream));
         } else {
@@ -441,7 +450,7 @@ int oc_cloth_step(oc_cloth c, int32_t nsteps) {
         }
         OC_HIP(oc::launch_cg(c->dc, s, c->stream, false));
     }
-    if (nsteps > 0) OC_HIP(oc::launch_update(c->dc, s, c->stream, true));
+    if (nsteps > 0 && s.warm) OC_HIP(oc::launch_update(c->dc, s, c->stream, true));
     c->assembled = false;
     return OC_OK;
 }

@@ -109,7 +109,9 @@ struct AssembleShared {
 
 // UPD: the window's x and v are the previous step's state before its
 // update; apply the update (k_update's arithmetic) on the fly and write the
-// tile's updated x, v to xo, vo (other blocks still read x and v).
+// tile's updated x, v to xo, vo (other blocks still read x and v).  Only
+// with a warm start: a cold start writes dv = 0 here, which other blocks
+// would still be reading.
 template <int TX, int TY, int WA, bool UPD>
 __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) {
     constexpr int NT = TX * TY;

@@ -59,11 +59,11 @@ def test_split_and_integrate_equal_step():
 
 @pytest.mark.parametrize("warm,tol_mode", [(True, 0), (False, 1), (True, 2)])
 def test_multi_step_call_equals_single_steps(o32, warm, tol_mode):
-    """oc_cloth_step(n) applies each step's update inside the next step's
-    assembly; it equals n calls of oc_cloth_step(1) and the oracle bit for
-    bit, with the ground clamp acting (the cloth starts just above the
-    ground and falls onto it), cold and warm CG starts and each stopping
-    rule."""
+    """With a warm CG start oc_cloth_step(n) applies each step's update
+    inside the next step's assembly (a cold start keeps an update launch per
+    step); either way n-step calls equal n calls of oc_cloth_step(1) and the
+    oracle bit for bit, with the ground clamp acting (the cloth starts just
+    above the ground and falls onto it) and each stopping rule."""
     from opencloth_amd.cloth import ImplicitCloth
 
     sc, X = scene()
