@@ -23,6 +23,7 @@ state resident in HBM.  Prints one JSON line:
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -49,12 +50,12 @@ def step_bytes(N: int, S: int, P: int, K: int) -> int:
     return (61 * N + 12 * S + 32 * P) + K * (160 * N + 32 * P) + 60 * N
 
 
-def read_traffic(path: str):
-    """HBM bytes per cg_spmv launch from the committed PMC summary, or None."""
+def read_profile_value(path: str, key: str):
+    """A `key = value` line of a committed profile summary, or None."""
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("cg_spmv_hbm_bytes_per_launch"):
+                if line.startswith(key):
                     return float(line.split("=")[1].split()[0])
     except OSError:
         pass
@@ -267,21 +268,36 @@ def bench_batch(steps: int, warmup: int, n_cloths: int = 4096, plain_steps: int 
     ps = ss - (nxs + nys)
     step_b = int(np.sum(61 * ns + 12 * ss + 32 * ps + it.astype(np.int64) * (160 * ns + 32 * ps) + 60 * ns))
 
-    plain = [ImplicitCloth.from_scene(sc.cloth(c)) for c in range(sc.n_cloths)]
-    for cl in plain:
-        cl.step(1)
-    for cl in plain:
-        cl.synchronize()
-    p0, p1 = Event(), Event()
-    p0.record()
-    for _ in range(plain_steps):
-        for cl in plain:
+    def plain_form(persistent: bool) -> float:
+        """ms per step of the same cloths stepped one cloth per launch
+        sequence (oc_cloth_step); persistent: each solve is one cooperative
+        launch that stops at convergence, so no CG launch is issued after a
+        cloth has converged."""
+        cls_ = []
+        for c in range(sc.n_cloths):
+            cs = sc.cloth(c)
+            p = dataclasses.replace(cs.params, cg_persistent=persistent)
+            cls_.append(ImplicitCloth(cs.nx, cs.ny, cs.X, cs.springs, cs.pinned, p, cs.V))
+        for cl in cls_:
             cl.step(1)
-    p1.record()
-    p1.synchronize()
-    ms_plain = p0.elapsed_ms(p1) / plain_steps
-    for cl in plain:
-        cl.close()
+        for cl in cls_:
+            cl.synchronize()
+        p0, p1 = Event(), Event()
+        p0.record()
+        for _ in range(plain_steps):
+            for cl in cls_:
+                cl.step(1)
+        p1.record()
+        p1.synchronize()
+        for cl in cls_:
+            cl.close()
+        return p0.elapsed_ms(p1) / plain_steps
+
+    ms_plain = plain_form(False)
+    ms_plain_p = plain_form(True)
+    traffic = read_profile_value(os.path.join(ROOT, "profiles", "batch_step_4096_cloths_pmc.txt"),
+                                 "batch_step_hbm_bytes_per_launch")
+    step_s = ms / steps * 1e-3
     return {
         "workload": f"{sc.n_cloths} cloths, sides U[16,128], {N} vertices, {len(sc.springs)} springs, "
                     "CG <= 40 it, rel tol 1e-4, dt 1/60",
@@ -293,9 +309,20 @@ def bench_batch(steps: int, warmup: int, n_cloths: int = 4096, plain_steps: int 
         "status_nonzero": int((st != 0).sum()),
         "step_algorithmic_bytes": step_b,
         "step_roofline_frac": step_b / (ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "roofline": {
+            "bound": "hbm", "kernel": "k_batch_step (forward)",
+            "achieved": step_b / step_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": step_b / step_s / 1e9 / HBM_PEAK_GBS,
+            "algorithmic_bytes_per_launch": step_b, "avg_launch_us": step_s * 1e6,
+            "traffic": traffic,
+            "traffic_GBps": None if traffic is None else traffic / step_s / 1e9,
+            "traffic_frac": None if traffic is None else traffic / step_s / 1e9 / HBM_PEAK_GBS,
+        },
         "plain_ms_per_step": ms_plain,
         "plain_vertex_steps_per_s": N / (ms_plain * 1e-3),
         "speedup_vs_plain": ms_plain / (ms / steps),
+        "plain_persistent_ms_per_step": ms_plain_p,
+        "speedup_vs_plain_persistent": ms_plain_p / (ms / steps),
         "setup_s": {"scene": t1 - t0, "create": t2 - t1},
     }
 
@@ -334,23 +361,34 @@ def bench_batch_adjoint(n_cloths: int = 512, steps: int = 60, every: int = 10,
     ms = e0.elapsed_ms(e1)
     b.close()
 
-    plain = [ImplicitCloth.from_scene(sc.cloth(c)) for c in range(sc.n_cloths)]
-    for c, cl in enumerate(plain[:2]):
-        cl.rollout_forward(steps, every)
-        cl.rollout_backward(np.zeros((cl.N, 3), np.float32))
-    p0, p1 = Event(), Event()
-    p0.record()
-    for c, cl in enumerate(plain):
-        v0, n = int(sc.voff[c]), cl.N
-        cl.set_state(sc.X[v0:v0 + n], sc.V[v0:v0 + n])
-        cl.rollout_forward(steps, every)
-        xt = cl.X
-        cl.rollout_backward(2.0 * (xt - target[v0:v0 + n]))
-    p1.record()
-    p1.synchronize()
-    ms_plain = p0.elapsed_ms(p1)
-    for cl in plain:
-        cl.close()
+    def plain_form(persistent: bool) -> float:
+        """ms of the same rollouts and adjoints one cloth at a time
+        (oc_cloth_rollout_forward/backward); persistent: every solve is one
+        cooperative launch that stops at convergence."""
+        cls_ = []
+        for c in range(sc.n_cloths):
+            cs = sc.cloth(c)
+            p = dataclasses.replace(cs.params, cg_persistent=persistent)
+            cls_.append(ImplicitCloth(cs.nx, cs.ny, cs.X, cs.springs, cs.pinned, p, cs.V))
+        for cl in cls_[:2]:
+            cl.rollout_forward(steps, every)
+            cl.rollout_backward(np.zeros((cl.N, 3), np.float32))
+        p0, p1 = Event(), Event()
+        p0.record()
+        for c, cl in enumerate(cls_):
+            v0, n = int(sc.voff[c]), cl.N
+            cl.set_state(sc.X[v0:v0 + n], sc.V[v0:v0 + n])
+            cl.rollout_forward(steps, every)
+            xt = cl.X
+            cl.rollout_backward(2.0 * (xt - target[v0:v0 + n]))
+        p1.record()
+        p1.synchronize()
+        for cl in cls_:
+            cl.close()
+        return p0.elapsed_ms(p1)
+
+    ms_plain = plain_form(False)
+    ms_plain_p = plain_form(True)
 
     from oracle import Oracle, make_params
 
@@ -374,9 +412,41 @@ def bench_batch_adjoint(n_cloths: int = 512, steps: int = 60, every: int = 10,
         "vertex_steps_per_s_fwd_adj": N * steps / (ms * 1e-3),
         "plain_ms_fwd_adj": ms_plain,
         "speedup_vs_plain": ms_plain / ms,
+        "plain_persistent_ms_fwd_adj": ms_plain_p,
+        "speedup_vs_plain_persistent": ms_plain_p / ms,
         "schedule": {"block_threads": sched[0], "vertices_per_thread": sched[1]},
         "grad_error_vs_fp64": errs,
+        # the adjoint launch (k_batch_step<..., true>) from the committed
+        # kernel trace and PMC summary of bench.py's batch_adjoint leg
+        "adjoint_kernel": adjoint_kernel_profile(),
     }
+
+
+def adjoint_kernel_profile() -> dict:
+    path = os.path.join(ROOT, "profiles", "batch_adjoint_512_cloths_pmc.txt")
+    us = read_profile_value(path, "batch_adjoint_avg_launch_us")
+    traffic = read_profile_value(path, "batch_adjoint_hbm_bytes_per_launch")
+    if us is None or traffic is None:
+        return {"avg_launch_us": us, "traffic": traffic}
+    return {"avg_launch_us": us, "traffic": traffic, "traffic_GBps": traffic / (us * 1e-6) / 1e9,
+            "traffic_frac": traffic / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, "source": path[len(ROOT) + 1:]}
+
+
+LEGS = ("large", "batch", "batch_adjoint", "entries", "cpu")
+
+
+def kernel_split(cl, steps: int, K: int, reps: int = 200) -> dict:
+    """Per-step kernel time of an n-step oc_cloth_step call from back-to-back
+    launches of each kernel class (oc_cloth_time_kernel): the first step's
+    plain assembly, n - 1 assemblies with the previous update fused in,
+    K (spmv + axpy) per step and one update per call."""
+    t = {k: cl.time_kernel(k, reps) for k in cl.KERNELS}
+    asm = (t["assemble"] + (steps - 1) * t["assemble_fused"]) / steps
+    out = {"assemble": asm * 1e-3, "cg_spmv": K * t["cg_spmv"] * 1e-3, "cg_axpy": K * t["cg_axpy"] * 1e-3,
+           "update": t["update"] / steps * 1e-3}
+    out["sum"] = sum(out.values())
+    out["launch_us"] = t
+    return out
 
 
 def main():
@@ -385,8 +455,10 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="large", choices=["large", "plain"])
+    ap.add_argument("--legs", default=",".join(LEGS),
+                    help="comma list of legs to run: " + ", ".join(LEGS) + " (the full line needs all)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched legs")
     ap.add_argument("--no-entries", action="store_true",
                     help="skip the Verlet / PBD / StepPhysics timings")
     ap.add_argument("--batch-steps", type=int, default=20)
@@ -399,76 +471,90 @@ def main():
               "2-row halo exchanges and two scalar all-reduces per CG iteration, a batch by "
               "instance, and neither is built here", file=sys.stderr)
         sys.exit(2)
+    legs = [x for x in args.legs.split(",") if x]
+    bad = [x for x in legs if x not in LEGS]
+    if bad:
+        ap.error(f"unknown legs {bad}")
+    if args.no_cpu:
+        legs = [x for x in legs if x != "cpu"]
+    if args.no_batch:
+        legs = [x for x in legs if x not in ("batch", "batch_adjoint")]
+    if args.no_entries:
+        legs = [x for x in legs if x != "entries"]
 
     from opencloth_amd import scenes
     from opencloth_amd.cloth import ImplicitCloth
     from opencloth_amd.hiptime import Event
 
     sc = scenes.large_scene() if args.workload == "large" else scenes.plain_scene()
-    cl = ImplicitCloth.from_scene(sc)
-    N, S, P, K = sc.n_vertices, cl.S, cl.P, sc.params.cg_max_iters
+    out = {"metric": METRIC}
+    if "large" in legs:
+        cl = ImplicitCloth.from_scene(sc)
+        N, S, P, K = sc.n_vertices, cl.S, cl.P, sc.params.cg_max_iters
 
-    cl.step(args.warmup)
-    cl.synchronize()
-    e0, e1 = Event(), Event()
-    e0.record()
-    cl.step(args.steps)
-    e1.record()
-    e1.synchronize()
-    ms = e0.elapsed_ms(e1)
-    it, st = cl.last_status()
-    ms_per_step = ms / args.steps
+        cl.step(args.warmup)
+        cl.synchronize()
+        e0, e1 = Event(), Event()
+        e0.record()
+        cl.step(args.steps)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1)
+        it, st = cl.last_status()
+        sticky = cl.sticky_status()
+        ms_per_step = ms / args.steps
 
-    prof = cl.profile_steps(min(args.steps, 20))
-    # the dominant kernel's launch time: back-to-back launches between two
-    # events (an event after every launch, as profile_steps does for the
-    # per-kernel breakdown, adds its own gap to each launch)
-    spmv_avg_s = cl.time_cg_spmv(300) * 1e-6
-    achieved = spmv_bytes(N, P) / spmv_avg_s / 1e9
-    traffic = read_traffic(os.path.join(ROOT, "profiles", "cg_spmv_512x512_pmc.txt"))
-
-    out = {
-        "metric": METRIC,
-        "value": N * args.steps / (ms * 1e-3),
-        "unit": "vertex-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": f"implicit step, {sc.nx}x{sc.ny} cloth, "
-                        + ("top edge pinned, CG fixed 30 iterations, dt 1/120"
-                           if args.workload == "large" else "2 corners pinned, CG <=10 it tol 1e-3, dt 1/60"),
-            "vertices": N, "springs": S, "pairs": P, "cg_iters": K,
-            "step_algorithmic_bytes": step_bytes(N, S, P, K),
-            "step_roofline_frac": step_bytes(N, S, P, K) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "last_cg_iters": it, "last_status": st,
-            "kernel_ms_per_step": {k: v[0] / max(min(args.steps, 20), 1) for k, v in prof.items()},
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "cg_spmv",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "algorithmic_bytes_per_launch": spmv_bytes(N, P),
-            "avg_launch_us": spmv_avg_s * 1e6,
-            "traffic_GBps": None if traffic is None else traffic / spmv_avg_s / 1e9,
-        },
-    }
-    if not args.no_batch:
+        split = kernel_split(cl, args.steps, K)
+        # the dominant kernel's launch time: back-to-back launches between two
+        # events (an event after every launch adds its own gap to each launch)
+        spmv_avg_s = cl.time_cg_spmv(300) * 1e-6
+        achieved = spmv_bytes(N, P) / spmv_avg_s / 1e9
+        traffic = read_profile_value(os.path.join(ROOT, "profiles", "cg_spmv_512x512_pmc.txt"),
+                                     "cg_spmv_hbm_bytes_per_launch")
+        out.update({
+            "value": N * args.steps / (ms * 1e-3),
+            "unit": "vertex-steps/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": ms_per_step,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f32",
+            "data": "synthetic",
+            "config": {
+                "workload": f"implicit step, {sc.nx}x{sc.ny} cloth, "
+                            + ("top edge pinned, CG fixed 30 iterations, dt 1/120"
+                               if args.workload == "large" else "2 corners pinned, CG <=10 it tol 1e-3, dt 1/60"),
+                "vertices": N, "springs": S, "pairs": P, "cg_iters": K,
+                "step_algorithmic_bytes": step_bytes(N, S, P, K),
+                "step_roofline_frac": step_bytes(N, S, P, K) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "last_cg_iters": it, "last_status": st, "sticky_status": sticky,
+                "kernel_ms_per_step": split,
+            },
+            "roofline": {
+                "bound": "hbm",
+                "kernel": "cg_spmv",
+                "achieved": achieved,
+                "peak": HBM_PEAK_GBS,
+                "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": traffic,
+                "algorithmic_bytes_per_launch": spmv_bytes(N, P),
+                "avg_launch_us": spmv_avg_s * 1e6,
+                "traffic_GBps": None if traffic is None else traffic / spmv_avg_s / 1e9,
+                "traffic_frac": None if traffic is None else traffic / spmv_avg_s / 1e9 / HBM_PEAK_GBS,
+            },
+        })
+        cl.close()
+    if "batch" in legs:
         out["batch"] = bench_batch(args.batch_steps, 2)
+    if "batch_adjoint" in legs:
         out["batch_adjoint"] = bench_batch_adjoint()
-    if not args.no_entries:
+    if "entries" in legs:
         out["entries"] = bench_entries()
-    if not args.no_cpu:
+    if "cpu" in legs:
         out["cpu_baseline"] = cpu_baseline(sc)
     if args.copy_probe > 0:
         out["copy_probe_bytes"] = copy_probe(args.copy_probe)

@@ -161,6 +161,12 @@ int oc_cloth_step(oc_cloth cloth, int32_t nsteps);
 int oc_cloth_synchronize(oc_cloth cloth);
 /* Status of the last solve: CG iterations and OC_SOLVE_* bits; synchronises. */
 int oc_cloth_last_status(oc_cloth cloth, int32_t* iters, int32_t* status);
+/* OR of the OC_SOLVE_* bits of every solve and state update since the last
+ * call of this function (or since creation), then cleared: a breakdown,
+ * non-convergence or non-finite state in any step of a multi-step call,
+ * step_physics or rollout is kept here even when a later step is clean.
+ * The reference's CG (IE:227-249) reports nothing; synchronises. */
+int oc_cloth_sticky_status(oc_cloth cloth, int32_t* status);
 
 /* Measurement (no counterpart in the reference).  Average duration of the
  * CG SpMV launch: assemble at the current state, run one CG iteration, then
@@ -169,6 +175,23 @@ int oc_cloth_last_status(oc_cloth cloth, int32_t* iters, int32_t* status);
  * are unchanged; dv and the CG vectors are left as that iteration leaves
  * them.  Needs cg_max_iters >= 2; synchronises. */
 int oc_cloth_time_cg_spmv(oc_cloth cloth, int32_t reps, float* avg_us);
+
+/* Kernel classes of oc_cloth_time_kernel. */
+#define OC_KERNEL_ASSEMBLE 0       /* assembly + CG start */
+#define OC_KERNEL_ASSEMBLE_FUSED 1 /* the same with the previous step's update applied
+                                      (steps after the first of oc_cloth_step) */
+#define OC_KERNEL_CG_SPMV 2        /* CG direction update and q = S A d */
+#define OC_KERNEL_CG_AXPY 3        /* CG dv, r update and r.r */
+#define OC_KERNEL_UPDATE 4         /* v += dv, x += dt v */
+/* Measurement (no counterpart in the reference).  Average duration of one
+ * kernel class: assemble at the current state and run one CG iteration,
+ * then time `reps` back-to-back launches of the class (the CG kernels of
+ * the second iteration, with the stopping test off) between two HIP events
+ * on the handle's stream.  x, v, dv and the status words are unchanged (the
+ * launches that write them write copies); the other CG vectors are left as
+ * the launches leave them.  Needs cg_max_iters >= 2, and a warm start for
+ * OC_KERNEL_ASSEMBLE_FUSED; synchronises. */
+int oc_cloth_time_kernel(oc_cloth cloth, int32_t kernel, int32_t reps, float* avg_us);
 
 /* ---- post-step passes of StepPhysics (IE:774-779) ----------------------- */
 /* Ellipsoid collider: transform and inverse as column-major 4x4 matrices
@@ -402,6 +425,10 @@ int oc_batch_step_eager(oc_batch batch, int32_t nsteps);
 /* CG iterations and OC_SOLVE_* bits of every cloth's last step (or, after
  * oc_batch_rollout_backward, of its last adjoint solve); synchronises. */
 int oc_batch_get_status(oc_batch batch, int32_t* iters, int32_t* status);
+/* Per cloth (n_cloths ints), the OR of the OC_SOLVE_* bits of every step
+ * and adjoint solve since the last call of this function (or since
+ * creation), then cleared; synchronises. */
+int oc_batch_sticky_status(oc_batch batch, int32_t* status);
 /* Per-spring stiffness of all cloths (concatenated spring lists). */
 int oc_batch_set_spring_stiffness(oc_batch batch, const float* ks);
 int oc_batch_get_spring_stiffness(oc_batch batch, float* ks);

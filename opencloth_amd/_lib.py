@@ -48,8 +48,10 @@ SIGNATURES = {
     "oc_cloth_step": (C.c_int, [_P, _I]),
     "oc_cloth_synchronize": (C.c_int, [_P]),
     "oc_cloth_last_status": (C.c_int, [_P, _P, _P]),
+    "oc_cloth_sticky_status": (C.c_int, [_P, _P]),
     "oc_cloth_profile_steps": (C.c_int, [_P, _I, _P, _P]),
     "oc_cloth_time_cg_spmv": (C.c_int, [_P, _I, _P]),
+    "oc_cloth_time_kernel": (C.c_int, [_P, _I, _I, _P]),
     "oc_cloth_set_spring_stiffness": (C.c_int, [_P, _P]),
     "oc_cloth_get_spring_stiffness": (C.c_int, [_P, _P]),
     "oc_cloth_rollout_forward": (C.c_int, [_P, _I, _I]),
@@ -96,6 +98,7 @@ SIGNATURES = {
     "oc_batch_step": (C.c_int, [_P, _I]),
     "oc_batch_step_eager": (C.c_int, [_P, _I]),
     "oc_batch_get_status": (C.c_int, [_P, _P, _P]),
+    "oc_batch_sticky_status": (C.c_int, [_P, _P]),
     "oc_batch_synchronize": (C.c_int, [_P]),
     "oc_batch_set_spring_stiffness": (C.c_int, [_P, _P]),
     "oc_batch_get_spring_stiffness": (C.c_int, [_P, _P]),

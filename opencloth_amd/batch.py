@@ -95,6 +95,13 @@ class ClothBatch:
         check(lib().oc_batch_get_status(self.h, ptr(it), ptr(st)))
         return it, st
 
+    def sticky_status(self):
+        """Per cloth, the OR of the status bits of every step since the last
+        call (then cleared)."""
+        st = np.zeros(self.n, np.int32)
+        check(lib().oc_batch_sticky_status(self.h, ptr(st)))
+        return st
+
     def synchronize(self):
         check(lib().oc_batch_synchronize(self.h))
 

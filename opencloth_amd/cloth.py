@@ -162,10 +162,26 @@ class ImplicitCloth:
         check(lib().oc_cloth_time_cg_spmv(self.h, int(reps), C.byref(t)))
         return t.value
 
+    KERNELS = ("assemble", "assemble_fused", "cg_spmv", "cg_axpy", "update")
+
+    def time_kernel(self, kernel: str, reps: int = 200) -> float:
+        """Average microseconds of one launch of a kernel class over reps
+        back-to-back launches (oc_cloth_time_kernel)."""
+        t = C.c_float()
+        check(lib().oc_cloth_time_kernel(self.h, self.KERNELS.index(kernel), int(reps), C.byref(t)))
+        return t.value
+
     def last_status(self):
         it = C.c_int32(); st = C.c_int32()
         check(lib().oc_cloth_last_status(self.h, C.byref(it), C.byref(st)))
         return it.value, st.value
+
+    def sticky_status(self) -> int:
+        """OR of the status bits of every solve and update since the last
+        call (then cleared)."""
+        st = C.c_int32()
+        check(lib().oc_cloth_sticky_status(self.h, C.byref(st)))
+        return st.value
 
     # -- post-step passes (StepPhysics, IE:774-779) ---------------------------
     def ellipsoid_collision(self, ell: dict):

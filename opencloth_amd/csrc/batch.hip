@@ -416,6 +416,7 @@ __global__ __launch_bounds__(NT, 4) void k_batch_step(DevBatch c, StepScalars s,
         if (act && lt == 0) {
             c.inst_iters[ins] = sit[g];
             c.inst_status[ins] = sst[g];
+            c.inst_sticky[ins] |= sst[g];  // one workgroup per cloth and launch
         }
         __syncthreads();
     }

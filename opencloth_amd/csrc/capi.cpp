@@ -147,6 +147,63 @@ int download_vec(oc_cloth_s* c, const float* src, float* host_aos) {
 
 }  // namespace
 
+namespace oc {
+
+size_t cloth_partial_floats(const oc_cloth_s* c) {
+    const size_t it = (size_t)std::max(c->prm.cg_max_iters, 1), nt = (size_t)c->dc.nt;
+    return (2 * it + 1) * nt + it + 2;
+}
+
+int cloth_step(oc_cloth_s* c, int nsteps, float* partials) {
+    const StepScalars s = scalars(c, c->prm.dt);
+    DevCloth d = c->dc;
+    if (partials) {
+        const size_t it = (size_t)std::max(s.max_it, 1), nt = (size_t)d.nt;
+        d.del_part = partials;
+        d.sig_part = partials + (it + 1) * nt;
+        d.scal = partials + (2 * it + 1) * nt;
+    }
+    // With a warm CG start (and at least one CG iteration), steps after the
+    // first apply the previous step's update inside their assembly (one
+    // launch fewer per step, the same bits); the last step's update, which
+    // also records the solve's status, is a launch of its own.  A cold start
+    // is not fused: its assembly zeroes dv, which neighbouring blocks of a
+    // fused assembly would still be reading for the update.
+    const bool fuse = s.warm && s.max_it >= 1;
+    bool pending = false;  // a solve whose update has not been launched yet
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < nsteps; k++) {
+        if (pending) {
+            DevCloth u = d;
+            u.xo = c->alt_x;
+            u.vo = c->alt_v;
+            if ((e = launch_assemble(u, s, c->stream, true)) != hipSuccess) break;
+            std::swap(d.x, c->alt_x);
+            std::swap(d.v, c->alt_v);
+            pending = false;
+        } else if ((e = launch_assemble(d, s, c->stream)) != hipSuccess) {
+            break;
+        }
+        if ((e = launch_cg(d, s, c->stream, false)) != hipSuccess) break;
+        if (fuse) {
+            pending = true;
+            continue;
+        }
+        if ((e = launch_update(d, s, c->stream, true)) != hipSuccess) break;
+    }
+    // The last solve's update; after a failed launch, the update of the last
+    // complete solve, so that x and v are never left one update behind it.
+    const hipError_t eu = pending ? launch_update(d, s, c->stream, true) : hipSuccess;
+    c->dc.x = d.x;
+    c->dc.v = d.v;
+    c->assembled = false;
+    if (e == hipSuccess) e = eu;
+    if (e != hipSuccess) return fail(OC_ERR_DEVICE, std::string("step launch failed: ") + hipGetErrorString(e));
+    return OC_OK;
+}
+
+}  // namespace oc
+
 extern "C" {
 
 const char* oc_version(void) { return "opencloth_amd 0.1 (gfx950)"; }
@@ -424,34 +481,17 @@ int oc_cloth_integrate_implicit(oc_cloth c, float dt) {
 int oc_cloth_step(oc_cloth c, int32_t nsteps) {
     if (!c) return fail(OC_ERR_INVALID_ARGUMENT, "cloth is NULL");
     if (nsteps < 0) return fail(OC_ERR_INVALID_ARGUMENT, "nsteps < 0");
-    const oc::StepScalars s = scalars(c, c->prm.dt);
-    // With a warm CG start, steps after the first apply the previous step's
-    // update inside their assembly (one launch fewer per step, the same
-    // bits); the last step's update, which also records the solve's status,
-    // is a launch of its own.  A cold start is not fused: its assembly zeroes
-    // dv, which neighbouring blocks of a fused assembly would still be
-    // reading for the update.
-    for (int k = 0; k < nsteps; k++) {
-        if (!s.warm) {
-            OC_HIP(oc::launch_assemble(c->dc, s, c->stream));
-            OC_HIP(oc::launch_cg(c->dc, s, c->stream, false));
-            OC_HIP(oc::launch_update(c->dc, s, c->stream, true));
-            continue;
-        }
-        if (k == 0) {
-            OC_HIP(oc::launch_assemble(c->dc, s, c->stream));
-        } else {
-            oc::DevCloth d = c->dc;
-            d.xo = c->alt_x;
-            d.vo = c->alt_v;
-            OC_HIP(oc::launch_assemble(d, s, c->stream, true));
-            std::swap(c->dc.x, c->alt_x);
-            std::swap(c->dc.v, c->alt_v);
-        }
-        OC_HIP(oc::launch_cg(c->dc, s, c->stream, false));
-    }
-    if (nsteps > 0 && s.warm) OC_HIP(oc::launch_update(c->dc, s, c->stream, true));
-    c->assembled = false;
+    return oc::cloth_step(c, nsteps, nullptr);
+}
+
+int oc_cloth_sticky_status(oc_cloth c, int32_t* status) {
+    if (!c || !status) return fail(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    OC_HIP(hipMemcpyAsync(c->ctl_host, c->dc.ctl, sizeof(int) * 8, hipMemcpyDeviceToHost,
+                          c->stream));
+    OC_HIP(hipMemsetAsync(c->dc.ctl + 3, 0, sizeof(int), c->stream));
+    OC_HIP(hipStreamSynchronize(c->stream));
+    *status = c->ctl_host[3];
+    if (c->ctl_host[4]) return fail(OC_ERR_DEVICE, "persistent CG grid barrier timed out");
     return OC_OK;
 }
 
@@ -521,6 +561,74 @@ int oc_cloth_time_cg_spmv(oc_cloth c, int32_t reps, float* avg_us) {
     (void)hipEventDestroy(e1);
     c->assembled = false;
     if (!ok) return fail(OC_ERR_DEVICE, "timed SpMV launches failed");
+    *avg_us = 1000.0f * ms / (float)reps;
+    return OC_OK;
+}
+
+int oc_cloth_time_kernel(oc_cloth c, int32_t kernel, int32_t reps, float* avg_us) {
+    if (!c || !avg_us) return fail(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (reps < 1) return fail(OC_ERR_INVALID_ARGUMENT, "reps < 1");
+    if (kernel < OC_KERNEL_ASSEMBLE || kernel > OC_KERNEL_UPDATE)
+        return fail(OC_ERR_INVALID_ARGUMENT, "unknown kernel class");
+    oc::StepScalars s = scalars(c, c->prm.dt);
+    if (s.max_it < 2) return fail(OC_ERR_INVALID_ARGUMENT, "cg_max_iters must be >= 2");
+    if (kernel == OC_KERNEL_ASSEMBLE_FUSED && !s.warm)
+        return fail(OC_ERR_INVALID_ARGUMENT, "the fused assembly needs a warm CG start");
+    s.tol_mode = OC_TOL_NONE;  // every timed launch does its full work
+    const size_t v3 = sizeof(float) * 3 * (size_t)c->dc.N;
+    // the status words, and x, v, dv: the timed launches write copies
+    OC_HIP(hipMemcpyAsync(c->ctl_host, c->dc.ctl, sizeof(int) * 8, hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(hipStreamSynchronize(c->stream));
+    int saved[8];
+    std::memcpy(saved, c->ctl_host, sizeof(saved));
+    oc::DevCloth d = c->dc;
+    OC_HIP(oc::launch_assemble(d, s, c->stream));
+    OC_HIP(oc::launch_cg_spmv(d, s, 0, c->stream));
+    OC_HIP(oc::launch_cg_axpy(d, s, 0, c->stream));
+    OC_HIP(oc::launch_cg_spmv(d, s, 1, c->stream));
+    if (kernel == OC_KERNEL_CG_AXPY) {  // dv += alpha d, r -= alpha q on copies
+        OC_HIP(hipMemcpyAsync(c->alt_x, d.dv, v3, hipMemcpyDeviceToDevice, c->stream));
+        OC_HIP(hipMemcpyAsync(c->alt_v, d.r, v3, hipMemcpyDeviceToDevice, c->stream));
+        d.dv = c->alt_x;
+        d.r = c->alt_v;
+    } else if (kernel == OC_KERNEL_UPDATE) {  // x, v updated in place on copies
+        OC_HIP(hipMemcpyAsync(c->alt_x, d.x, v3, hipMemcpyDeviceToDevice, c->stream));
+        OC_HIP(hipMemcpyAsync(c->alt_v, d.v, v3, hipMemcpyDeviceToDevice, c->stream));
+        d.x = c->alt_x;
+        d.v = c->alt_v;
+    } else if (kernel == OC_KERNEL_ASSEMBLE_FUSED) {  // updated x, v go to the other set
+        d.xo = c->alt_x;
+        d.vo = c->alt_v;
+    }
+    auto launch = [&]() -> hipError_t {
+        switch (kernel) {
+            case OC_KERNEL_ASSEMBLE: return oc::launch_assemble(d, s, c->stream);
+            case OC_KERNEL_ASSEMBLE_FUSED: return oc::launch_assemble(d, s, c->stream, true);
+            case OC_KERNEL_CG_SPMV: return oc::launch_cg_spmv(d, s, 1, c->stream);
+            case OC_KERNEL_CG_AXPY: return oc::launch_cg_axpy(d, s, 1, c->stream);
+            default: return oc::launch_update(d, s, c->stream);
+        }
+    };
+    OC_HIP(launch());  // warm-up
+    hipEvent_t e0, e1;
+    OC_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(OC_ERR_DEVICE, "hipEventCreate failed");
+    }
+    bool ok = hipEventRecord(e0, c->stream) == hipSuccess;
+    for (int r = 0; ok && r < reps; r++) ok = launch() == hipSuccess;
+    float ms = 0.0f;
+    ok = ok && hipEventRecord(e1, c->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+         hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::memcpy(c->ctl_host, saved, sizeof(saved));
+    ok = ok && hipMemcpyAsync(c->dc.ctl, c->ctl_host, sizeof(int) * 8, hipMemcpyHostToDevice, c->stream) ==
+                   hipSuccess &&
+         hipStreamSynchronize(c->stream) == hipSuccess;
+    c->assembled = false;
+    if (!ok) return fail(OC_ERR_DEVICE, "timed launches failed");
     *avg_us = 1000.0f * ms / (float)reps;
     return OC_OK;
 }

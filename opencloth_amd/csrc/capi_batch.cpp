@@ -269,7 +269,7 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
     const size_t slotc = align_up(sizeof(float) * 3 * (size_t)std::max(wa, 1) * N);
     const size_t pin = align_up(N);
     const size_t per = align_up(sizeof(int) * n_cloths);
-    const size_t total = 8 * vec + blocks + as + am + pm + pk + 3 * sp + slotc + pin + 5 * per + 256;
+    const size_t total = 8 * vec + blocks + as + am + pm + pk + 3 * sp + slotc + pin + 6 * per + 256;
     if (hipMalloc(&b->arena, total) != hipSuccess) {
         delete b;
         return set_error(OC_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(total) + " bytes failed");
@@ -309,6 +309,7 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
     d.inst_n = (const int*)take(per);
     d.inst_iters = (int*)take(per);
     d.inst_status = (int*)take(per);
+    d.inst_sticky = (int*)take(per);
     d.queue = (int*)take(256);
 
     auto fail_free = [&](int code, const std::string& msg) {
@@ -499,6 +500,15 @@ int oc_batch_get_status(oc_batch b, int32_t* iters, int32_t* status) {
         OC_HIP(hipMemcpy(iters, b->dc.inst_iters, sizeof(int) * b->n, hipMemcpyDeviceToHost));
     if (status)
         OC_HIP(hipMemcpy(status, b->dc.inst_status, sizeof(int) * b->n, hipMemcpyDeviceToHost));
+    return OC_OK;
+}
+
+int oc_batch_sticky_status(oc_batch b, int32_t* status) {
+    if (!b || !status) return set_error(OC_ERR_INVALID_ARGUMENT, "NULL argument");
+    OC_HIP(hipStreamSynchronize(b->stream));
+    OC_HIP(hipMemcpy(status, b->dc.inst_sticky, sizeof(int) * b->n, hipMemcpyDeviceToHost));
+    OC_HIP(hipMemsetAsync(b->dc.inst_sticky, 0, sizeof(int) * b->n, b->stream));
+    OC_HIP(hipStreamSynchronize(b->stream));
     return OC_OK;
 }
 

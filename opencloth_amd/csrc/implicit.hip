@@ -77,6 +77,35 @@ __global__ __launch_bounds__(TX* TY) void k_forces(DevCloth c, StepScalars s) {
     st3(c.f, N, i, acc);
 }
 
+// Iteration count and status of the solve, by all threads of one block
+// (uniform control flow: partial_sum synchronises the block).  Bits are
+// OR-ed in, so update blocks may flag non-finite values concurrently.
+__device__ inline void cg_finish_block(const DevCloth& c, const StepScalars& s, float* bc) {
+    const int done = c.ctl[0];
+    int it;
+    float delta;
+    if (done) {
+        it = c.ctl[1];
+        delta = c.scal[it];
+    } else {
+        it = s.max_it;
+        delta = partial_sum(c.del_part + (size_t)it * c.nt, c.nt, bc);
+    }
+    // r0.r0: written by the first spmv phase, or equal to delta without iterations
+    const float delta0 = s.max_it > 0 ? c.scal[0] : delta;
+    if (threadIdx.x == 0) {
+        const float thresh = s.tol_mode == 1 ? s.tol2 * delta0 : s.tol2;
+        const int st = c.ctl[2];
+        int extra = 0;
+        if (s.tol_mode != 2 && delta > thresh && !(st & OC_SOLVE_BREAKDOWN))
+            extra |= OC_SOLVE_NOT_CONVERGED;
+        if (!isfinite(delta)) extra |= OC_SOLVE_NONFINITE;
+        c.ctl[1] = it;
+        atomicOr(c.ctl + 2, extra);
+        atomicOr(c.ctl + 3, st | extra);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // assembly + CG start
 // ---------------------------------------------------------------------------
@@ -105,6 +134,7 @@ struct AssembleShared {
     float wv[3][WY][WX];
     float wdv[3][WY][WX];
     float ws[TX * TY / 64];
+    float bc;
 };
 
 // UPD: the window's x and v are the previous step's state before its
@@ -124,10 +154,16 @@ __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) 
     const int gi = bx * TX + lx, gj = by * TY + ly;
     const bool own = gi < c.nx && gj < c.ny;
     const int N = c.N, i = gj * c.nx + gi;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        c.ctl[0] = 0;
-        c.ctl[1] = 0;
-        c.ctl[2] = 0;
+    if (blockIdx.x == 0) {
+        // the previous solve's iteration count and status bits go into the
+        // sticky word before the control block is cleared (with UPD that
+        // solve had no update launch to record them)
+        if (UPD) cg_finish_block(c, s, &sh.bc);
+        if (threadIdx.x == 0) {
+            c.ctl[0] = 0;
+            c.ctl[1] = 0;
+            c.ctl[2] = 0;
+        }
     }
     // own slot tables and the first springs' constants, then the window
     int sps[WA];
@@ -188,6 +224,11 @@ __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) 
         if (UPD) {
             st3o(c.xo, N, i, xi);
             st3o(c.vo, N, i, vi);
+            // k_update's check of the updated state; the bit belongs to the
+            // previous step, whose status is already in the sticky word
+            if (!(isfinite(xi.x) && isfinite(xi.y) && isfinite(xi.z) && isfinite(vi.x) &&
+                  isfinite(vi.y) && isfinite(vi.z)))
+                atomicOr(c.ctl + 3, OC_SOLVE_NONFINITE);
         }
         V3 acc = {0.0f, 0.0f, 0.0f}, kv = {0.0f, 0.0f, 0.0f};
         if (!pinned) acc = {0.0f + s.mass * s.gx, 0.0f + s.mass * s.gy, 0.0f + s.mass * s.gz};
@@ -565,35 +606,6 @@ __global__ void k_reset_barrier(unsigned* bar) {
     if (threadIdx.x == 0) *bar = 0u;
 }
 
-// Iteration count and status of the solve, by all threads of one block
-// (uniform control flow: partial_sum synchronises the block).  Bits are
-// OR-ed in, so update blocks may flag non-finite values concurrently.
-__device__ inline void cg_finish_block(const DevCloth& c, const StepScalars& s, float* bc) {
-    const int done = c.ctl[0];
-    int it;
-    float delta;
-    if (done) {
-        it = c.ctl[1];
-        delta = c.scal[it];
-    } else {
-        it = s.max_it;
-        delta = partial_sum(c.del_part + (size_t)it * c.nt, c.nt, bc);
-    }
-    // r0.r0: written by the first spmv phase, or equal to delta without iterations
-    const float delta0 = s.max_it > 0 ? c.scal[0] : delta;
-    if (threadIdx.x == 0) {
-        const float thresh = s.tol_mode == 1 ? s.tol2 * delta0 : s.tol2;
-        const int st = c.ctl[2];
-        int extra = 0;
-        if (s.tol_mode != 2 && delta > thresh && !(st & OC_SOLVE_BREAKDOWN))
-            extra |= OC_SOLVE_NOT_CONVERGED;
-        if (!isfinite(delta)) extra |= OC_SOLVE_NONFINITE;
-        c.ctl[1] = it;
-        atomicOr(c.ctl + 2, extra);
-        atomicOr(c.ctl + 3, st | extra);
-    }
-}
-
 // Iteration count and status of the solve (one 64-thread block).
 __global__ void k_cg_finish(DevCloth c, StepScalars s) {
     __shared__ float bc;
@@ -728,6 +740,11 @@ hipError_t launch_forces(const DevCloth& c, const StepScalars& s, hipStream_t st
 }
 
 hipError_t launch_assemble(const DevCloth& c, const StepScalars& s, hipStream_t st, bool fused_update) {
+    // A fused update needs a warm start (a cold assembly zeroes dv, which
+    // neighbouring blocks still read for the update) and at least one CG
+    // iteration (the previous solve's status is taken from partials that a
+    // solve without iterations leaves where this assembly writes its own).
+    if (fused_update && (!s.warm || s.max_it < 1)) return hipErrorInvalidValue;
     const dim3 grid(c.nt), block(kTileX * kTileY);
     if (c.wa <= 16) {
         if (fused_update)

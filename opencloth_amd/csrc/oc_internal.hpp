@@ -116,6 +116,12 @@ struct DevCloth {
     float* vo;
 };
 
+// Implicit steps of a cloth handle, the body of oc_cloth_step (capi.cpp).
+// partials: NULL for the handle's own CG partials and scalars, else a device
+// buffer of cloth_partial_floats(c) floats used in their place.
+size_t cloth_partial_floats(const ::oc_cloth_s* c);
+int cloth_step(::oc_cloth_s* c, int nsteps, float* partials);
+
 // Blocks of the persistent CG launch (all resident at once), 0 if unsupported.
 int persistent_cg_grid(const DevCloth& c);
 
@@ -287,6 +293,7 @@ struct DevBatch {
     int* queue;
     int* inst_iters;
     int* inst_status;
+    int* inst_sticky;  // OR of every step's status bits since the last query
 };
 
 // Buffers of one batched adjoint step (SoA, stride Ntot).

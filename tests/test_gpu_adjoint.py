@@ -86,7 +86,7 @@ def test_gradient_against_fp64_adjoint(c4):
     ek = np.linalg.norm(c4["gks"] - gk64) / np.linalg.norm(gk64)
     ev = np.linalg.norm(c4["gv0"] - gv64) / np.linalg.norm(gv64)
     say(f"relative gradient error vs FP64 adjoint: ks {ek:.3e}, v0 {ev:.3e}")
-    assert ek < 1e-3 and ev < 1e-3
+    assert ek < 1e-4 and ev < 1e-4
 
 
 @pytest.mark.parametrize("eps,bar", [("1e-05", 1e-3), ("0.001", 2e-2)])

@@ -118,7 +118,8 @@ def test_cg_solution_bitwise(o32, warm):
 
 def test_plain_rollout_300_steps(o32, o64):
     """21x21, 300 steps: per-step max relative position error <= 1e-4 against
-    the FP32 restatement; FP64 error reported."""
+    the FP32 restatement (it is 0: the run is bit-exact) and <= 1e-5 against
+    the FP64 restatement (about 1.2e-6 measured)."""
     sc = scenes.plain_scene()
     say("rollout 21x21 x300")
     cl = make_cloth(sc)
@@ -136,7 +137,7 @@ def test_plain_rollout_300_steps(o32, o64):
         err64[k] = rel_pos_err(X, h64[k])
     say("max rel err vs fp32 %.3e, vs fp64 %.3e" % (err32.max(), err64.max()))
     assert err32.max() <= 1e-4
-    assert err64.max() <= 1e-3
+    assert err64.max() <= 1e-5
 
 
 def test_large_scene_two_steps(o32):
