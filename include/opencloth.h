@@ -157,6 +157,18 @@ int oc_cloth_integrate_implicit(oc_cloth cloth, float dt);
  * Replaces StepPhysics (IE:774-779) for the implicit integrator; the
  * reference's post-step collision and Provot passes are separate calls. */
 int oc_cloth_step(oc_cloth cloth, int32_t nsteps);
+/* oc_cloth_step that allocates nothing.  No counterpart in the reference:
+ * for callers that cannot allocate device memory around the step (HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The same
+ * launches and results as oc_cloth_step, but the CG partials and scalars
+ * come from a fixed device array of 2^20 floats (4 MiB) that each GPU holds
+ * until the process ends, instead of the handle's own buffer.  Cloths whose
+ * partials need more (tiles x (2 cg_max_iters + 1) + cg_max_iters + 2
+ * floats, tiles = ceil(nx/64) * ceil(ny/8)) are rejected with
+ * OC_ERR_INVALID_ARGUMENT.  The cloth's device must be current.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams or threads) race. */
+int oc_cloth_step_noalloc(oc_cloth cloth, int32_t nsteps);
 /* Wait for the handle's stream. */
 int oc_cloth_synchronize(oc_cloth cloth);
 /* Status of the last solve: CG iterations and OC_SOLVE_* bits; synchronises. */

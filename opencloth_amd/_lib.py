@@ -46,6 +46,7 @@ SIGNATURES = {
     "oc_cloth_update": (C.c_int, [_P, _F]),
     "oc_cloth_integrate_implicit": (C.c_int, [_P, _F]),
     "oc_cloth_step": (C.c_int, [_P, _I]),
+    "oc_cloth_step_noalloc": (C.c_int, [_P, _I]),
     "oc_cloth_synchronize": (C.c_int, [_P]),
     "oc_cloth_last_status": (C.c_int, [_P, _P, _P]),
     "oc_cloth_sticky_status": (C.c_int, [_P, _P]),

@@ -149,6 +149,8 @@ int download_vec(oc_cloth_s* c, const float* src, float* host_aos) {
 
 namespace oc {
 
+int cloth_device(const oc_cloth_s* c) { return c->device; }
+
 size_t cloth_partial_floats(const oc_cloth_s* c) {
     const size_t it = (size_t)std::max(c->prm.cg_max_iters, 1), nt = (size_t)c->dc.nt;
     return (2 * it + 1) * nt + it + 2;

@@ -120,6 +120,7 @@ struct DevCloth {
 // partials: NULL for the handle's own CG partials and scalars, else a device
 // buffer of cloth_partial_floats(c) floats used in their place.
 size_t cloth_partial_floats(const ::oc_cloth_s* c);
+int cloth_device(const ::oc_cloth_s* c);  // the device the handle was created on
 int cloth_step(::oc_cloth_s* c, int nsteps, float* partials);
 
 // Blocks of the persistent CG launch (all resident at once), 0 if unsupported.
