@@ -232,7 +232,7 @@ def cpu_baseline(sc, budget_s: float = 12.0, single_budget_s: float = 6.0) -> di
     }
 
 
-def bench_batch(steps: int, warmup: int, n_cloths: int = 4096, plain_steps: int = 1) -> dict:
+def bench_batch(steps: int, warmup: int, n_cloths: int = 4096, plain_steps: int = 1, plain: bool = True) -> dict:
     """Batched forward step: n_cloths cloths with sides uniform in [16,128],
     ks log-uniform [20,200], kd uniform [0.1,0.5], 2-8 random top-edge pins,
     CG <= 40 iterations with per-cloth relative tolerance 1e-4, dt 1/60, one
@@ -293,8 +293,8 @@ def bench_batch(steps: int, warmup: int, n_cloths: int = 4096, plain_steps: int 
             cl.close()
         return p0.elapsed_ms(p1) / plain_steps
 
-    ms_plain = plain_form(False)
-    ms_plain_p = plain_form(True)
+    ms_plain = plain_form(False) if plain else float("nan")
+    ms_plain_p = plain_form(True) if plain else float("nan")
     traffic = read_profile_value(os.path.join(ROOT, "profiles", "batch_step_4096_cloths_pmc.txt"),
                                  "batch_step_hbm_bytes_per_launch")
     step_s = ms / steps * 1e-3
@@ -327,7 +327,7 @@ def bench_batch(steps: int, warmup: int, n_cloths: int = 4096, plain_steps: int 
     }
 
 
-def bench_batch_adjoint(n_cloths: int = 512, steps: int = 60, every: int = 10,
+def bench_batch_adjoint(n_cloths: int = 512, steps: int = 60, every: int = 10, plain: bool = True,
                         sample=(0, 255, 511)) -> dict:
     """Batched forward + adjoint: n_cloths cloths, sides uniform in [32,96],
     per-spring ks log-uniform [20,200], top edges pinned, 60-step rollouts to
@@ -387,8 +387,8 @@ def bench_batch_adjoint(n_cloths: int = 512, steps: int = 60, every: int = 10,
             cl.close()
         return p0.elapsed_ms(p1)
 
-    ms_plain = plain_form(False)
-    ms_plain_p = plain_form(True)
+    ms_plain = plain_form(False) if plain else float("nan")
+    ms_plain_p = plain_form(True) if plain else float("nan")
 
     from oracle import Oracle, make_params
 
@@ -435,17 +435,19 @@ def adjoint_kernel_profile() -> dict:
 LEGS = ("large", "batch", "batch_adjoint", "entries", "cpu")
 
 
-def kernel_split(cl, steps: int, K: int, reps: int = 200) -> dict:
-    """Per-step kernel time of an n-step oc_cloth_step call from back-to-back
-    launches of each kernel class (oc_cloth_time_kernel): the first step's
-    plain assembly, n - 1 assemblies with the previous update fused in,
-    K (spmv + axpy) per step and one update per call."""
-    t = {k: cl.time_kernel(k, reps) for k in cl.KERNELS}
-    asm = (t["assemble"] + (steps - 1) * t["assemble_fused"]) / steps
-    out = {"assemble": asm * 1e-3, "cg_spmv": K * t["cg_spmv"] * 1e-3, "cg_axpy": K * t["cg_axpy"] * 1e-3,
-           "update": t["update"] / steps * 1e-3}
+def kernel_split(cl, steps: int, reps: int = 200) -> dict:
+    """Per-step kernel time of an n-step oc_cloth_step call, measured inside
+    the real launch sequence: oc_cloth_profile_steps runs the same steps with
+    a start/stop event attached to every kernel dispatch (no packets between
+    launches) and sums each class.  Beside it, each class's duration in
+    back-to-back launches of that kernel alone (oc_cloth_time_kernel), which
+    reuses its inputs from cache and reads lower."""
+    prof = cl.profile_steps(steps)
+    out = {k: v[0] / steps for k, v in prof.items()}
     out["sum"] = sum(out.values())
-    out["launch_us"] = t
+    out["launches_per_step"] = {k: v[1] / steps for k, v in prof.items()}
+    out["in_sequence_launch_us"] = {k: (1e3 * v[0] / v[1] if v[1] else None) for k, v in prof.items()}
+    out["back_to_back_launch_us"] = {k: cl.time_kernel(k, reps) for k in cl.KERNELS}
     return out
 
 
@@ -462,6 +464,8 @@ def main():
     ap.add_argument("--no-entries", action="store_true",
                     help="skip the Verlet / PBD / StepPhysics timings")
     ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the batched legs' one-cloth-per-launch denominators (profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the timing, run one device-to-device copy of MIB MiB (a kernel of "
                          "known HBM traffic, the scale for FETCH_SIZE/WRITE_SIZE counters)")
@@ -504,10 +508,11 @@ def main():
         sticky = cl.sticky_status()
         ms_per_step = ms / args.steps
 
-        split = kernel_split(cl, args.steps, K)
-        # the dominant kernel's launch time: back-to-back launches between two
-        # events (an event after every launch adds its own gap to each launch)
-        spmv_avg_s = cl.time_cg_spmv(300) * 1e-6
+        # the same steps again with every dispatch timed (same state path)
+        split = kernel_split(cl, args.steps)
+        # the dominant kernel's launch time inside the step's launch sequence
+        spmv_avg_s = split["in_sequence_launch_us"]["cg_spmv"] * 1e-6
+        spmv_b2b_s = cl.time_cg_spmv(300) * 1e-6
         achieved = spmv_bytes(N, P) / spmv_avg_s / 1e9
         traffic = read_profile_value(os.path.join(ROOT, "profiles", "cg_spmv_512x512_pmc.txt"),
                                      "cg_spmv_hbm_bytes_per_launch")
@@ -543,15 +548,33 @@ def main():
                 "traffic": traffic,
                 "algorithmic_bytes_per_launch": spmv_bytes(N, P),
                 "avg_launch_us": spmv_avg_s * 1e6,
+                "avg_launch_us_back_to_back": spmv_b2b_s * 1e6,
+                "frac_back_to_back": spmv_bytes(N, P) / spmv_b2b_s / 1e9 / HBM_PEAK_GBS,
                 "traffic_GBps": None if traffic is None else traffic / spmv_avg_s / 1e9,
                 "traffic_frac": None if traffic is None else traffic / spmv_avg_s / 1e9 / HBM_PEAK_GBS,
             },
         })
         cl.close()
+        # the same workload with the single-reduction CG (one launch per CG
+        # iteration; Chronopoulos-Gear recurrence, other rounding)
+        p1 = dataclasses.replace(sc.params, cg_single_reduction=True)
+        cl1 = ImplicitCloth(sc.nx, sc.ny, sc.X, sc.springs, sc.pinned, p1, sc.V)
+        cl1.step(args.warmup)
+        cl1.synchronize()
+        e0.record()
+        cl1.step(args.steps)
+        e1.record()
+        e1.synchronize()
+        ms1 = e0.elapsed_ms(e1) / args.steps
+        out["config"]["cg_single_reduction"] = {
+            "ms_per_step": ms1, "vertex_steps_per_s": N / (ms1 * 1e-3),
+            "speedup_vs_standard": ms_per_step / ms1,
+            "last_status": cl1.last_status(), "kernel_ms_per_step": kernel_split(cl1, args.steps)}
+        cl1.close()
     if "batch" in legs:
-        out["batch"] = bench_batch(args.batch_steps, 2)
+        out["batch"] = bench_batch(args.batch_steps, 2, plain=not args.no_plain)
     if "batch_adjoint" in legs:
-        out["batch_adjoint"] = bench_batch_adjoint()
+        out["batch_adjoint"] = bench_batch_adjoint(plain=not args.no_plain)
     if "entries" in legs:
         out["entries"] = bench_entries()
     if "cpu" in legs:

@@ -63,6 +63,20 @@ typedef enum {
 #define OC_FLAG_JACOBI 4        /* Jacobi-preconditioned CG: Baraff and Witkin's modified PCG
                                    with P = diag(A), the intent of BW:308-330 and BW:784-785;
                                    the stopping tests then measure r.(P^-1 r) */
+#define OC_FLAG_CG_SINGLE_REDUCTION 16 /* the CG recurrence of Chronopoulos and Gear (1989) with one
+                                   reduction point per iteration: both dots of an iteration come
+                                   from one launch, so an iteration is one launch instead of two.
+                                   The same solution in exact arithmetic, other rounding; the
+                                   default keeps the reference's recurrence (IE:227-249).
+                                   Single cloths only (batches ignore it); takes precedence over
+                                   OC_FLAG_CG_PERSISTENT */
+#define OC_FLAG_LIST_ORDER 8    /* oc_pbd_params.flags: project the constraints in list order,
+                                   one after another as UpdateInternalConstraints does
+                                   (PBD:876-889), instead of colour by colour: one workgroup
+                                   runs them level by level, a constraint's level being one
+                                   more than the highest level among the earlier constraints
+                                   that share a particle with it (the reference's results;
+                                   for demo-sized cloths) */
 
 /* Spring record, identical to the reference's `struct Spring` (IE:68-73). */
 typedef struct {
@@ -84,6 +98,10 @@ typedef struct {
     int32_t cg_tol_mode; /* OC_TOL_* */
     int32_t warm_start;  /* nonzero: CG starts from the previous step's dv (IE:691, BW:788) */
     int32_t flags;       /* OC_FLAG_* */
+    float damping;       /* velocity damping c: every free vertex gets the force c*v, as
+                            BW:706-708 adds DEFAULT_DAMPING*V (negative damps; 0 in IE,
+                            which comments it out at IE:655); its Jacobian c*I enters
+                            A = M - h df/dv - h^2 df/dx, which BW:737-772 omits */
 } oc_params;
 
 /* ---- library ----------------------------------------------------------- */
@@ -163,8 +181,8 @@ int oc_cloth_step(oc_cloth cloth, int32_t nsteps);
  * launches and results as oc_cloth_step, but the CG partials and scalars
  * come from a fixed device array of 2^20 floats (4 MiB) that each GPU holds
  * until the process ends, instead of the handle's own buffer.  Cloths whose
- * partials need more (tiles x (2 cg_max_iters + 1) + cg_max_iters + 2
- * floats, tiles = ceil(nx/64) * ceil(ny/8)) are rejected with
+ * partials need more (2 (tiles + 1)(cg_max_iters + 1) + 2 floats,
+ * tiles = ceil(nx/64) * ceil(ny/8)) are rejected with
  * OC_ERR_INVALID_ARGUMENT.  The cloth's device must be current.  Every call
  * on one GPU shares that GPU's array: calls that overlap on one GPU (other
  * streams or threads) race. */
@@ -343,7 +361,8 @@ int oc_pbd_destroy(oc_pbd pbd);
 int oc_pbd_set_stream(oc_pbd pbd, void* hip_stream);
 int oc_pbd_set_state(oc_pbd pbd, const float* x, const float* v);
 int oc_pbd_get_state(oc_pbd pbd, float* x, float* v);
-/* Colours of the distance and bending constraints. */
+/* Colours of the distance and bending constraints (with OC_FLAG_LIST_ORDER,
+ * their levels). */
 int oc_pbd_num_colors(oc_pbd pbd, int32_t* n_distance_colors, int32_t* n_bending_colors);
 /* nsteps of StepPhysics (PBD:915-925): gravity, global and centre-of-mass
  * damping, prediction (PBD:601-668), solver_iterations passes of distance
@@ -373,11 +392,12 @@ int oc_cloth_rollout_backward(oc_cloth cloth, const float* grad_x, const float* 
                               float* grad_ks, float* grad_x0, float* grad_v0);
 
 /* Measurement hook (no counterpart in the reference, whose only timing is a
- * frame counter, IE:562-584): runs nsteps like oc_cloth_step with a HIP
- * event after every kernel launch and returns, per kernel class
- * (0 assemble, 1 cg_spmv, 2 cg_axpy, 3 cg_finish, 4 update), the summed
- * milliseconds between consecutive events and the launch counts.
- * ms[5], launches[5]; synchronises. */
+ * frame counter, IE:562-584): runs nsteps exactly as oc_cloth_step does
+ * (same launches, same results), each kernel launched with a start and a
+ * stop event attached to its own dispatch, and returns per kernel class
+ * (0 assembly, 1 first CG launch of an iteration (cg_spmv), 2 second
+ * (cg_axpy), 3 other solver launches, 4 update) the summed kernel
+ * milliseconds and the launch counts.  ms[5], launches[5]; synchronises. */
 int oc_cloth_profile_steps(oc_cloth cloth, int32_t nsteps, float* ms, int32_t* launches);
 
 /* Introspection of the assembled system (for tests and tools).

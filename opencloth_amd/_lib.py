@@ -19,7 +19,8 @@ OC_ERR_OUT_OF_MEMORY = -4
 OC_ERR_NO_DEVICE = -5
 
 SOLVE_NOT_CONVERGED, SOLVE_BREAKDOWN, SOLVE_NONFINITE = 1, 2, 4
-OC_FLAG_GROUND_CLAMP, OC_FLAG_CG_PERSISTENT, OC_FLAG_JACOBI = 1, 2, 4
+OC_FLAG_GROUND_CLAMP, OC_FLAG_CG_PERSISTENT, OC_FLAG_JACOBI, OC_FLAG_LIST_ORDER = 1, 2, 4, 8
+OC_FLAG_CG_SINGLE_REDUCTION = 16
 
 # Every symbol include/opencloth.h declares, with (restype, argtypes).
 _P = C.c_void_p
@@ -114,7 +115,7 @@ class OcParams(C.Structure):
     _fields_ = [
         ("mass", C.c_float), ("gravity", C.c_float * 3), ("dt", C.c_float),
         ("cg_max_iters", C.c_int32), ("cg_tol", C.c_float), ("cg_tol_mode", C.c_int32),
-        ("warm_start", C.c_int32), ("flags", C.c_int32),
+        ("warm_start", C.c_int32), ("flags", C.c_int32), ("damping", C.c_float),
     ]
 
 

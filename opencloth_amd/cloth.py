@@ -29,7 +29,9 @@ def to_oc_params(p: StepParams) -> OcParams:
     q.warm_start = 1 if p.warm_start else 0
     q.flags = ((_lib.OC_FLAG_GROUND_CLAMP if p.ground_clamp else 0)
                | (_lib.OC_FLAG_CG_PERSISTENT if p.cg_persistent else 0)
-               | (_lib.OC_FLAG_JACOBI if p.cg_jacobi else 0))
+               | (_lib.OC_FLAG_JACOBI if p.cg_jacobi else 0)
+               | (_lib.OC_FLAG_CG_SINGLE_REDUCTION if getattr(p, "cg_single_reduction", False) else 0))
+    q.damping = getattr(p, "damping", 0.0)
     return q
 
 
@@ -150,7 +152,8 @@ class ImplicitCloth:
     KERNEL_CLASSES = ("assemble", "cg_spmv", "cg_axpy", "cg_finish", "update")
 
     def profile_steps(self, nsteps: int) -> dict:
-        """nsteps steps with an event after every launch: {kernel: (ms, launches)}."""
+        """nsteps steps exactly as step() runs them, each kernel dispatch
+        timed by its own start/stop events: {kernel class: (ms, launches)}."""
         ms = (C.c_float * 5)(); n = (C.c_int32 * 5)()
         check(lib().oc_cloth_profile_steps(self.h, int(nsteps), ms, n))
         return {k: (ms[i], n[i]) for i, k in enumerate(self.KERNEL_CLASSES)}
@@ -383,7 +386,8 @@ class PBDCloth:
         p.global_damping = self.params.global_damping
         p.k_damp = self.params.k_damp
         p.solver_iterations = self.params.solver_iterations
-        p.flags = _lib.OC_FLAG_GROUND_CLAMP if self.params.ground else 0
+        p.flags = ((_lib.OC_FLAG_GROUND_CLAMP if self.params.ground else 0)
+                   | (_lib.OC_FLAG_LIST_ORDER if getattr(self.params, "list_order", False) else 0))
         self._p = p
         dist = np.ascontiguousarray(distance, dtype=PBD_DISTANCE_DTYPE)
         bend = np.ascontiguousarray(bending, dtype=PBD_BENDING_DTYPE)

@@ -86,8 +86,10 @@ __device__ inline float assemble_vertex(const DevBatch& c, const StepScalars& s,
     if (!s.warm) st3(c.dv, N, i, dvi);
     V3 acc = {0.0f, 0.0f, 0.0f}, kv = {0.0f, 0.0f, 0.0f};
     if (!pinned) acc = {0.0f + s.mass * s.gx, 0.0f + s.mass * s.gy, 0.0f + s.mass * s.gz};
-    V3 adv = {s.mass * dvi.x, s.mass * dvi.y, s.mass * dvi.z};
-    V3 dg = {s.mass, s.mass, s.mass};  // diag(A) = m - sum of the pair blocks' diagonals
+    if (!pinned && s.damp != 0.0f)
+        acc = {acc.x + s.damp * vi.x, acc.y + s.damp * vi.y, acc.z + s.damp * vi.z};
+    V3 adv = {s.md * dvi.x, s.md * dvi.y, s.md * dvi.z};
+    V3 dg = {s.md, s.md, s.md};  // diag(A) = m - h c - sum of the pair blocks' diagonals
     float C[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     // Springs are evaluated from this vertex's side (x1 = x_i), which is
     // bitwise the negated p1-side force and K w; the next spring's data is
@@ -192,7 +194,7 @@ __device__ inline V3 spmv_stencil(const DevBatch& c, int i, int nx, V3 di, V3 ac
 __device__ inline float spmv_vertex(const DevBatch& c, const StepScalars& s, int i, int nx) {
     const int N = c.Ntot;
     const V3 di = ld3(c.d, N, i);
-    V3 acc = scl3(s.mass, di);
+    V3 acc = scl3(s.md, di);
     if (c.stencil) {
         acc = spmv_stencil(c, i, nx, di, acc);
         const V3 q = c.pin[i] ? V3{0.0f, 0.0f, 0.0f} : acc;
@@ -239,6 +241,8 @@ __device__ inline void adjoint_gather_vertex(const DevBatch& c, const StepScalar
              pi = ld3(a.dvsol, N, i);
     const V3 wi = add3(vi, pi);
     V3 gx = ld3(a.gx_out, N, i), gv = ld3(a.gvt, N, i);
+    if (s.damp != 0.0f && !c.pin[i])  // the damping force c v_i: dJ/dv_i += h c mu_i
+        gv = {gv.x + s.h * (s.damp * mi.x), gv.y + s.h * (s.damp * mi.y), gv.z + s.h * (s.damp * mi.z)};
     for (int k = 0; k < c.wa; k++) {
         const int sp = c.ainc_s[(size_t)k * N + i];
         if (sp < 0) break;

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -43,6 +44,7 @@ struct oc_cloth_s {
     oc::DevCloth dc{};
     float* alt_x = nullptr;  // the other (x, v) set: a fused update writes here,
     float* alt_v = nullptr;  // then the two sets swap
+    int32_t* spring_cst = nullptr;  // stencil assembly form: owner-plane index per spring
     bool assembled = false;
     // differentiable rollouts: checkpoints, one segment of states, adjoint vectors
     void* adj_arena = nullptr;
@@ -66,6 +68,8 @@ oc::StepScalars scalars(const oc_cloth_s* c, float dt) {
     s.tol_mode = c->prm.cg_tol_mode;
     s.warm = c->prm.warm_start ? 1 : 0;
     s.flags = c->prm.flags;
+    s.damp = c->prm.damping;
+    s.md = s.mass - s.h * s.damp;
     return s;
 }
 
@@ -80,6 +84,7 @@ int check_params(const oc_params* p) {
     if (p->cg_tol_mode < 0 || p->cg_tol_mode > 2)
         return fail(OC_ERR_INVALID_ARGUMENT, "unknown cg_tol_mode");
     if (!(p->cg_tol >= 0.0f)) return fail(OC_ERR_INVALID_ARGUMENT, "cg_tol must be >= 0");
+    if (!std::isfinite(p->damping)) return fail(OC_ERR_INVALID_ARGUMENT, "damping must be finite");
     return OC_OK;
 }
 
@@ -90,8 +95,9 @@ int ensure_scratch(oc_cloth_s* c, int max_iters) {
     const int need = std::max(max_iters, 1);
     if (c->scratch && need <= c->max_it_alloc) return OC_OK;
     const size_t nt = c->dc.nt;
-    const size_t del = align_up(sizeof(float) * (need + 1) * nt), sig = align_up(sizeof(float) * need * nt),
-                 scal = align_up(sizeof(float) * (need + 2));
+    // scal holds the reduced r.r values and (single-reduction CG) the alphas
+    const size_t del = align_up(sizeof(float) * (need + 1) * nt), sig = align_up(sizeof(float) * (need + 1) * nt),
+                 scal = align_up(sizeof(float) * 2 * (need + 2));
     const size_t bytes = del + sig + scal + align_up(sizeof(int) * 8);
     void* fresh = nullptr;
     if (hipMalloc(&fresh, bytes) != hipSuccess)
@@ -109,6 +115,7 @@ int ensure_scratch(oc_cloth_s* c, int max_iters) {
     c->dc.sig_part = (float*)(p + del);
     c->dc.scal = (float*)(p + del + sig);
     c->dc.ctl = (int*)(p + del + sig + scal);
+    c->dc.cg1_alpha = need + 2;
     c->max_it_alloc = need;
     OC_HIP(hipMemsetAsync(fresh, 0, bytes, c->stream));
     return OC_OK;
@@ -153,7 +160,7 @@ int cloth_device(const oc_cloth_s* c) { return c->device; }
 
 size_t cloth_partial_floats(const oc_cloth_s* c) {
     const size_t it = (size_t)std::max(c->prm.cg_max_iters, 1), nt = (size_t)c->dc.nt;
-    return (2 * it + 1) * nt + it + 2;
+    return 2 * (it + 1) * nt + 2 * (it + 2);
 }
 
 int cloth_step(oc_cloth_s* c, int nsteps, float* partials) {
@@ -163,7 +170,8 @@ int cloth_step(oc_cloth_s* c, int nsteps, float* partials) {
         const size_t it = (size_t)std::max(s.max_it, 1), nt = (size_t)d.nt;
         d.del_part = partials;
         d.sig_part = partials + (it + 1) * nt;
-        d.scal = partials + (2 * it + 1) * nt;
+        d.scal = partials + 2 * (it + 1) * nt;
+        d.cg1_alpha = (int)it + 2;
     }
     // With a warm CG start (and at least one CG iteration), steps after the
     // first apply the previous step's update inside their assembly (one
@@ -279,7 +287,7 @@ int oc_cloth_create(int32_t nx, int32_t ny, const float* x0, const float* v0,
     // the CG and assembly kernels address every array with 32-bit byte offsets
     const uint64_t largest = std::max<uint64_t>(
         {4ull * 6 * N * (uint64_t)std::max(t.n_planes, 1), 4ull * N * (uint64_t)std::max(t.wa, 1),
-         2ull * N * (uint64_t)std::max(t.wp, 1), 4ull * S, 12ull * N});
+         2ull * N * (uint64_t)std::max(t.wp, 1), 4ull * S, 12ull * N, 4ull * oc::kCstPlanes * N});
     if (largest >= (1ull << 32)) {
         delete c;
         return fail(OC_ERR_UNSUPPORTED_TOPOLOGY,
@@ -294,7 +302,14 @@ int oc_cloth_create(int32_t nx, int32_t ny, const float* x0, const float* v0,
     const size_t pair_k = align_up(sizeof(uint16_t) * std::max<size_t>(t.pair_mask.size(), 1));
     const size_t sp = align_up(sizeof(float) * S);
     const size_t pin = align_up(N);
-    const size_t total = 13 * vec + blocks + ainc_s + ainc_m + pair_m + pair_k + 3 * sp + pin;
+    // stencil assembly form (each spring evaluated once per tile), unless
+    // OPENCLOTH_ASSEMBLY=per-end asks for the per-end kernel (measurement)
+    const char* asm_env = std::getenv("OPENCLOTH_ASSEMBLY");
+    const bool st_asm = t.stencil_asm && !(asm_env && std::strcmp(asm_env, "per-end") == 0);
+    const size_t amask = st_asm ? align_up(sizeof(uint32_t) * N) : 0;
+    const size_t cst = st_asm ? align_up(sizeof(float) * oc::kCstPlanes * N) : 0;
+    const size_t scst = st_asm ? align_up(sizeof(int32_t) * S) : 0;
+    const size_t total = 17 * vec + blocks + ainc_s + ainc_m + pair_m + pair_k + 3 * sp + pin + amask + cst + scst;
     if (hipMalloc(&c->arena, total) != hipSuccess) {
         delete c;
         return fail(OC_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(total) + " bytes failed");
@@ -329,6 +344,14 @@ int oc_cloth_create(int32_t nx, int32_t ny, const float* x0, const float* v0,
     d.pin = (const uint8_t*)take(pin);
     c->alt_x = (float*)take(vec);  // last, so the other arrays keep their offsets
     c->alt_v = (float*)take(vec);
+    d.stencil_asm = st_asm ? 1 : 0;
+    d.asm_mask = st_asm ? (const uint32_t*)take(amask) : nullptr;
+    d.cst = st_asm ? (const float*)take(cst) : nullptr;
+    c->spring_cst = st_asm ? (int32_t*)take(scst) : nullptr;
+    d.r2 = (float*)take(vec);
+    d.q2 = (float*)take(vec);
+    d.s1 = (float*)take(vec);
+    d.s2 = (float*)take(vec);
 
     auto cleanup = [&](int code) {
         (void)hipFree(c->arena);
@@ -362,6 +385,12 @@ int oc_cloth_create(int32_t nx, int32_t ny, const float* x0, const float* v0,
               hipMemcpy((void*)d.ks, ks.data(), S * 4, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy((void*)d.kd, kd.data(), S * 4, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy((void*)d.pin, pins.data(), N, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && st_asm)
+        ok = hipMemcpy((void*)d.asm_mask, t.asm_mask.data(), N * 4, hipMemcpyHostToDevice) == hipSuccess &&
+             (t.S == 0 || hipMemcpy(c->spring_cst, t.spring_cst.data(), (size_t)t.S * 4,
+                                    hipMemcpyHostToDevice) == hipSuccess) &&
+             oc::launch_scatter_cst(d, c->spring_cst, t.S, 0, 3, nullptr) == hipSuccess &&
+             hipDeviceSynchronize() == hipSuccess;
     if (!ok) return cleanup(fail(OC_ERR_DEVICE, "upload of topology failed"));
     if (hipHostMalloc((void**)&c->ctl_host, sizeof(int) * 8) != hipSuccess)
         return cleanup(fail(OC_ERR_OUT_OF_MEMORY, "hipHostMalloc failed"));
@@ -501,41 +530,47 @@ int oc_cloth_profile_steps(oc_cloth c, int32_t nsteps, float* ms, int32_t* launc
     if (!c || !ms || !launches) return fail(OC_ERR_INVALID_ARGUMENT, "NULL argument");
     if (nsteps < 0) return fail(OC_ERR_INVALID_ARGUMENT, "nsteps < 0");
     const oc::StepScalars s = scalars(c, c->prm.dt);
-    const int per_step = 2 * s.max_it + 2;
-    std::vector<hipEvent_t> ev(per_step + 1);
-    std::vector<int> cls(per_step);
-    for (auto& e : ev) OC_HIP(hipEventCreate(&e));
+    // the launches oc_cloth_step makes: per step an assembly and the solver's
+    // launches (at most 2 per iteration, a barrier reset and a finish), one
+    // update per step or per call
+    const int cap = nsteps * (2 * s.max_it + 4) + 2;
+    std::vector<hipEvent_t> e0(cap), e1(cap);
+    std::vector<int> kind(cap);
+    int made = 0;
+    int rc = OC_OK;
+    for (; made < cap; made++) {
+        if (hipEventCreate(&e0[made]) != hipSuccess) break;
+        if (hipEventCreate(&e1[made]) != hipSuccess) {
+            (void)hipEventDestroy(e0[made]);
+            break;
+        }
+    }
+    if (made < cap) rc = fail(OC_ERR_DEVICE, "hipEventCreate failed");
+    oc::LaunchProfile lp{e0.data(), e1.data(), kind.data(), 0, cap};
+    if (rc == OC_OK) {
+        oc::set_launch_profile(&lp);
+        rc = oc::cloth_step(c, nsteps, nullptr);
+        oc::set_launch_profile(nullptr);
+    }
     for (int k = 0; k < 5; k++) {
         ms[k] = 0.0f;
         launches[k] = 0;
     }
-    int rc = OC_OK;
-    for (int step = 0; step < nsteps && rc == OC_OK; step++) {
-        int n = 0;
-        auto rec = [&](hipError_t e, int kind) {
-            if (e != hipSuccess) return false;
-            cls[n] = kind;
-            return hipEventRecord(ev[++n], c->stream) == hipSuccess;
-        };
-        bool ok = hipEventRecord(ev[0], c->stream) == hipSuccess &&
-                  rec(oc::launch_assemble(c->dc, s, c->stream), 0);
-        for (int k = 0; ok && k < s.max_it; k++)
-            ok = rec(oc::launch_cg_spmv(c->dc, s, k, c->stream), 1) &&
-                 rec(oc::launch_cg_axpy(c->dc, s, k, c->stream), 2);
-        ok = ok && rec(oc::launch_update(c->dc, s, c->stream, true), 4);
-        if (!ok || hipEventSynchronize(ev[n]) != hipSuccess) {
-            rc = fail(OC_ERR_DEVICE, "profiled launch failed");
+    if (rc == OC_OK && lp.n > 0 && hipEventSynchronize(e1[lp.n - 1]) != hipSuccess)
+        rc = fail(OC_ERR_DEVICE, "hipEventSynchronize failed");
+    for (int k = 0; rc == OC_OK && k < lp.n; k++) {
+        float t = 0.0f;
+        if (hipEventElapsedTime(&t, e0[k], e1[k]) != hipSuccess) {
+            rc = fail(OC_ERR_DEVICE, "hipEventElapsedTime failed");
             break;
         }
-        for (int k = 0; k < n; k++) {
-            float t = 0.0f;
-            (void)hipEventElapsedTime(&t, ev[k], ev[k + 1]);
-            ms[cls[k]] += t;
-            launches[cls[k]]++;
-        }
+        ms[kind[k]] += t;
+        launches[kind[k]]++;
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
-    c->assembled = false;
+    for (int k = 0; k < made; k++) {
+        (void)hipEventDestroy(e0[k]);
+        (void)hipEventDestroy(e1[k]);
+    }
     return rc;
 }
 
@@ -715,6 +750,7 @@ int oc_cloth_set_spring_stiffness(oc_cloth c, const float* ks) {
     if (!c || !ks) return fail(OC_ERR_INVALID_ARGUMENT, "NULL argument");
     OC_HIP(hipMemcpyAsync((void*)c->dc.ks, ks, sizeof(float) * c->topo.S, hipMemcpyHostToDevice,
                           c->stream));
+    if (c->dc.stencil_asm) OC_HIP(oc::launch_scatter_cst(c->dc, c->spring_cst, c->topo.S, 1, 1, c->stream));
     OC_HIP(hipStreamSynchronize(c->stream));
     c->roll_valid = false;
     return OC_OK;

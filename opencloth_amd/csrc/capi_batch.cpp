@@ -62,13 +62,15 @@ oc::StepScalars batch_scalars(const oc_batch_s* b) {
     s.tol_mode = b->prm.cg_tol_mode;
     s.warm = b->prm.warm_start ? 1 : 0;
     s.flags = b->prm.flags;
+    s.damp = b->prm.damping;
+    s.md = s.mass - s.h * s.damp;
     return s;
 }
 
 int check_params(const oc_params* p) {
     if (!p) return set_error(OC_ERR_INVALID_ARGUMENT, "params is NULL");
     if (!(p->mass > 0.0f) || !(p->dt > 0.0f) || p->cg_max_iters < 0 || p->cg_tol_mode < 0 ||
-        p->cg_tol_mode > 2 || !(p->cg_tol >= 0.0f))
+        p->cg_tol_mode > 2 || !(p->cg_tol >= 0.0f) || !std::isfinite(p->damping))
         return set_error(OC_ERR_INVALID_ARGUMENT, "invalid step parameters");
     return OC_OK;
 }

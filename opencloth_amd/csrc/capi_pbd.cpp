@@ -3,7 +3,11 @@
 // and bending constraint lists.  Constraints are coloured on the host
 // (greedy, in list order: a constraint takes the lowest colour not yet used
 // at any of its particles) and stored colour by colour; one launch projects
-// one colour.
+// one colour.  With OC_FLAG_LIST_ORDER they are stored level by level
+// instead (a constraint's level is one more than the highest level among the
+// earlier constraints sharing a particle with it) and one single-workgroup
+// launch projects all levels of a step in order: the reference's sequential
+// results.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,12 +33,27 @@ float k_prime(float k, int ns) {
     return kp > 1.0f ? 1.0f : kp;
 }
 
-// Greedy colouring in list order.  order gets the constraint ids colour by
-// colour, ptr the colour offsets.  Inside a colour the constraints are
+// Constraints grouped by group id col[c] (ncol groups): order gets the ids
+// group by group, ptr the group offsets.  Inside a group the constraints are
 // ordered by first particle: they touch disjoint particles, so the order
 // does not change the result, and neighbouring lanes then read neighbouring
 // particles (the reference lists column constraints column by column, which
 // in list order would put a whole grid row between neighbouring lanes).
+void bucket(int n, int arity, const std::vector<int32_t>& p, const std::vector<int>& col, int ncol,
+            std::vector<int32_t>& order, std::vector<int32_t>& ptr) {
+    ptr.assign((size_t)ncol + 1, 0);
+    for (int c = 0; c < n; c++) ptr[col[c] + 1]++;
+    for (int k = 0; k < ncol; k++) ptr[k + 1] += ptr[k];
+    std::vector<int32_t> fill(ptr.begin(), ptr.end());
+    order.assign((size_t)n, 0);
+    for (int c = 0; c < n; c++) order[fill[col[c]]++] = c;
+    for (int k = 0; k < ncol; k++)
+        std::stable_sort(order.begin() + ptr[k], order.begin() + ptr[k + 1], [&](int32_t x, int32_t y) {
+            return p[(size_t)arity * x] < p[(size_t)arity * y];
+        });
+}
+
+// Greedy colouring in list order (see bucket for the order inside a colour).
 // Returns the colour count, -1 past 64.
 int colour(int N, int n, int arity, const std::vector<int32_t>& p, std::vector<int32_t>& order,
            std::vector<int32_t>& ptr) {
@@ -50,17 +69,28 @@ int colour(int N, int n, int arity, const std::vector<int32_t>& p, std::vector<i
         for (int a = 0; a < arity; a++) used[p[(size_t)arity * c + a]] |= uint64_t(1) << k;
         ncol = std::max(ncol, k + 1);
     }
-    ptr.assign((size_t)ncol + 1, 0);
-    for (int c = 0; c < n; c++) ptr[col[c] + 1]++;
-    for (int k = 0; k < ncol; k++) ptr[k + 1] += ptr[k];
-    std::vector<int32_t> fill(ptr.begin(), ptr.end());
-    order.assign((size_t)n, 0);
-    for (int c = 0; c < n; c++) order[fill[col[c]]++] = c;
-    for (int k = 0; k < ncol; k++)
-        std::stable_sort(order.begin() + ptr[k], order.begin() + ptr[k + 1], [&](int32_t x, int32_t y) {
-            return p[(size_t)arity * x] < p[(size_t)arity * y];
-        });
+    bucket(n, arity, p, col, ncol, order, ptr);
     return ncol;
+}
+
+// List-order levels: constraint c's level is one more than the highest level
+// among the earlier constraints sharing a particle with it, so every earlier
+// constraint it depends on sits in a lower level and the constraints of one
+// level share no particle.  Returns the level count.
+int levels(int N, int n, int arity, const std::vector<int32_t>& p, std::vector<int32_t>& order,
+           std::vector<int32_t>& ptr) {
+    std::vector<int> last((size_t)N, -1);
+    std::vector<int> lev((size_t)n);
+    int nlev = 0;
+    for (int c = 0; c < n; c++) {
+        int l = 0;
+        for (int a = 0; a < arity; a++) l = std::max(l, last[p[(size_t)arity * c + a]] + 1);
+        lev[c] = l;
+        for (int a = 0; a < arity; a++) last[p[(size_t)arity * c + a]] = l;
+        nlev = std::max(nlev, l + 1);
+    }
+    bucket(n, arity, p, lev, nlev, order, ptr);
+    return nlev;
 }
 
 }  // namespace
@@ -175,8 +205,12 @@ int oc_pbd_create(int32_t n_vertices, const float* x0, const uint8_t* pinned,
     oc_pbd_s* p = new (std::nothrow) oc_pbd_s();
     if (!p) return set_error(OC_ERR_OUT_OF_MEMORY, "host allocation failed");
     std::vector<int32_t> d_order, b_order;
-    if (colour(N, n_distance, 2, dp, d_order, p->d_ptr) < 0 ||
-        colour(N, n_bending, 3, bp, b_order, p->b_ptr) < 0) {
+    const bool list_order = (params->flags & OC_FLAG_LIST_ORDER) != 0;
+    if (list_order) {
+        levels(N, n_distance, 2, dp, d_order, p->d_ptr);
+        levels(N, n_bending, 3, bp, b_order, p->b_ptr);
+    } else if (colour(N, n_distance, 2, dp, d_order, p->d_ptr) < 0 ||
+               colour(N, n_bending, 3, bp, b_order, p->b_ptr) < 0) {
         delete p;
         return set_error(OC_ERR_UNSUPPORTED_TOPOLOGY, "constraints need more than 64 colours");
     }
@@ -219,7 +253,8 @@ int oc_pbd_create(int32_t n_vertices, const float* x0, const uint8_t* pinned,
     const size_t sd = align_up(4 * std::max<size_t>(nd, 1)), sb = align_up(4 * std::max<size_t>(nb, 1));
     const size_t spart = align_up(sizeof(float) * oc::kPbdSums * nblk);
     const size_t ssum = align_up(sizeof(float) * oc::kPbdSums);
-    const size_t total = 3 * vec + sN + 4 * sd + 6 * sb + spart + ssum;
+    const size_t sdl = align_up(4 * p->d_ptr.size()), sbl = align_up(4 * p->b_ptr.size());
+    const size_t total = 3 * vec + sN + 4 * sd + 6 * sb + spart + ssum + sdl + sbl;
     if (hipMalloc(&p->arena, total) != hipSuccess) {
         delete p;
         return set_error(OC_ERR_OUT_OF_MEMORY, "hipMalloc failed");
@@ -249,6 +284,12 @@ int oc_pbd_create(int32_t n_vertices, const float* x0, const uint8_t* pinned,
     float* bbk = (float*)take(sb);
     d.part = (float*)take(spart);
     d.sums = (float*)take(ssum);
+    int32_t* dlv = (int32_t*)take(sdl);
+    int32_t* blv = (int32_t*)take(sbl);
+    d.d_lv = dlv;
+    d.b_lv = blv;
+    d.nd_lv = (int)p->d_ptr.size() - 1;
+    d.nb_lv = (int)p->b_ptr.size() - 1;
     d.w = w;
     d.d_p1 = dd1;
     d.d_p2 = dd2;
@@ -261,7 +302,9 @@ int oc_pbd_create(int32_t n_vertices, const float* x0, const uint8_t* pinned,
     d.b_w = bbw;
     d.b_kp = bbk;
     bool ok = hipMemset(p->arena, 0, total) == hipSuccess &&
-              hipMemcpy(w, W.data(), 4 * (size_t)N, hipMemcpyHostToDevice) == hipSuccess;
+              hipMemcpy(w, W.data(), 4 * (size_t)N, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(dlv, p->d_ptr.data(), 4 * p->d_ptr.size(), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(blv, p->b_ptr.data(), 4 * p->b_ptr.size(), hipMemcpyHostToDevice) == hipSuccess;
     if (ok && nd)
         ok = hipMemcpy(dd1, d1.data(), 4 * nd, hipMemcpyHostToDevice) == hipSuccess &&
              hipMemcpy(dd2, d2.data(), 4 * nd, hipMemcpyHostToDevice) == hipSuccess &&
@@ -336,13 +379,19 @@ int oc_pbd_step(oc_pbd p, int32_t nsteps, const oc_ellipsoid* e) {
     const oc::DevPbd& d = p->dev;
     const int nd_col = (int)p->d_ptr.size() - 1, nb_col = (int)p->b_ptr.size() - 1;
     for (int n = 0; n < nsteps; n++) {
-        OC_HIP(oc::launch_pbd_predict(d, s, p->stream));
-        for (int it = 0; it < p->prm.solver_iterations; it++) {
-            for (int c = 0; c < nd_col; c++)
-                OC_HIP(oc::launch_pbd_distance(d, p->d_ptr[c], p->d_ptr[c + 1] - p->d_ptr[c], p->stream));
-            for (int c = 0; c < nb_col; c++)
-                OC_HIP(oc::launch_pbd_bending(d, s, p->b_ptr[c], p->b_ptr[c + 1] - p->b_ptr[c], p->stream));
-            if (p->prm.flags & OC_FLAG_GROUND_CLAMP) OC_HIP(oc::launch_pbd_ground(d, p->stream));
+        if (p->prm.flags & OC_FLAG_LIST_ORDER) {
+            OC_HIP(oc::launch_pbd_predict_seq(d, s, p->stream));
+            OC_HIP(oc::launch_pbd_levels(d, s, p->prm.solver_iterations,
+                                         (p->prm.flags & OC_FLAG_GROUND_CLAMP) ? 1 : 0, p->stream));
+        } else {
+            OC_HIP(oc::launch_pbd_predict(d, s, p->stream));
+            for (int it = 0; it < p->prm.solver_iterations; it++) {
+                for (int c = 0; c < nd_col; c++)
+                    OC_HIP(oc::launch_pbd_distance(d, p->d_ptr[c], p->d_ptr[c + 1] - p->d_ptr[c], p->stream));
+                for (int c = 0; c < nb_col; c++)
+                    OC_HIP(oc::launch_pbd_bending(d, s, p->b_ptr[c], p->b_ptr[c + 1] - p->b_ptr[c], p->stream));
+                if (p->prm.flags & OC_FLAG_GROUND_CLAMP) OC_HIP(oc::launch_pbd_ground(d, p->stream));
+            }
         }
         if (e) OC_HIP(oc::launch_ellipsoid(d.t, d.v, d.N, *e, 0, p->stream));
         OC_HIP(oc::launch_pbd_integrate(d, s, p->stream));

@@ -26,6 +26,8 @@
 // Every CG kernel issues all loads that do not depend on the previous
 // reduction (its own vectors, pair blocks, the partials) before waiting, so
 // the cross-tile reduction overlaps the memory traffic.
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 
 #include "oc_device.hpp"
@@ -59,6 +61,7 @@ __global__ __launch_bounds__(TX* TY) void k_forces(DevCloth c, StepScalars s) {
     if (!c.pin[i]) {
         V3 xi = ld3(c.x, N, i), vi = ld3(c.v, N, i);
         acc = {0.0f + s.mass * s.gx, 0.0f + s.mass * s.gy, 0.0f + s.mass * s.gz};
+        if (s.damp != 0.0f) acc = {acc.x + s.damp * vi.x, acc.y + s.damp * vi.y, acc.z + s.damp * vi.z};
         for (int k = 0; k < c.wa; k++) {
             const int sp = c.ainc_s[k * N + i];
             if (sp < 0) break;
@@ -232,8 +235,10 @@ __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) 
         }
         V3 acc = {0.0f, 0.0f, 0.0f}, kv = {0.0f, 0.0f, 0.0f};
         if (!pinned) acc = {0.0f + s.mass * s.gx, 0.0f + s.mass * s.gy, 0.0f + s.mass * s.gz};
-        V3 adv = {s.mass * dvi.x, s.mass * dvi.y, s.mass * dvi.z};
-        V3 dg = {s.mass, s.mass, s.mass};  // diag(A) = m - sum of the pair blocks' diagonals
+        if (!pinned && s.damp != 0.0f)
+            acc = {acc.x + s.damp * vi.x, acc.y + s.damp * vi.y, acc.z + s.damp * vi.z};
+        V3 adv = {s.md * dvi.x, s.md * dvi.y, s.md * dvi.z};
+        V3 dg = {s.md, s.md, s.md};  // diag(A) = m - h c - sum of the pair blocks' diagonals
         float C[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         // Every spring is evaluated from this vertex's side (x1 = x_i): when
         // i is the spring's p2 that negates dp, f and K(v_i - v_j) exactly
@@ -276,6 +281,301 @@ __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) 
                 for (int q = 0; q < 6; q++) C[q] = 0.0f;
             }
         }
+        V3 b;
+        b.x = pinned ? 0.0f : h * (acc.x + h * kv.x);
+        b.y = pinned ? 0.0f : h * (acc.y + h * kv.y);
+        b.z = pinned ? 0.0f : h * (acc.z + h * kv.z);
+        st3o(c.b, N, i, b);
+        V3 r = pinned ? V3{0.0f, 0.0f, 0.0f} : V3{b.x - adv.x, b.y - adv.y, b.z - adv.z};
+        st3o(c.r, N, i, r);
+        if (s.flags & OC_FLAG_JACOBI) {
+            const V3 pv = {jacobi_inv(dg.x), jacobi_inv(dg.y), jacobi_inv(dg.z)};
+            st3o(c.pinv, N, i, pv);
+            const V3 z = {pv.x * r.x, pv.y * r.y, pv.z * r.z};
+            st3o(c.s, N, i, z);
+            st3o(c.d0, N, i, z);
+            t = dot3(r, z);
+        } else {
+            st3o(c.d0, N, i, r);
+            t = dot3(r, r);
+        }
+    }
+    float p = tile_sum<NT>(t, sh.ws);
+    if (threadIdx.x == 0) c.del_part[tile] = p;
+}
+
+// ---------------------------------------------------------------------------
+// assembly, stencil assembly form: every spring evaluated once per tile
+// ---------------------------------------------------------------------------
+// A pair's owner is its lower-index vertex; the partner sits at one of six
+// offsets (dx, dy) from it (owned direction o, owned_k).  The assembly visits
+// the owned directions one after another.  In each, the block evaluates the
+// springs owned by its tile's vertices and by the owners just outside the
+// tile whose partner is inside (the direction's "strip"), once each, from
+// the owner's side; the results (force, K (v_owner - v_partner) and the
+// pair-block term) go to LDS.  Then every vertex adds, in its own stencil
+// order, the pair it is partner of (the owner's results negated) and the
+// pair it owns.  Negating the owner's force and K w equals evaluating from
+// the partner's side (dp, f and K w change sign exactly, K and P are even in
+// dp) up to the sign of zero, and these sums start at +0, where a signed
+// zero cannot change a sum; so the results are the per-end kernel's bits.
+// A second spring of a pair (the reference's duplicated last bends) is
+// evaluated by each end itself.
+// One spring from the x1 end: force on x1, K (v1 - v2) and the pair-block
+// term h (kd P) + h^2 K, in the per-end kernel's expressions.
+struct SpringOut {
+    V3 f, kw;
+    float c[6];
+};
+__device__ inline SpringOut eval_spring(V3 x1, V3 x2, V3 v1, V3 v2, float L0, float ks, float kd,
+                                        float h, float hh) {
+    SpringOut o;
+    float K[6], P[6];
+    spring_terms(x1, x2, v1, v2, L0, ks, kd, o.f, K, P);
+#pragma unroll
+    for (int q = 0; q < 6; q++) o.c[q] = h * (kd * P[q]) + hh * K[q];
+    o.kw = symmv(K, V3{v1.x - v2.x, v1.y - v2.y, v1.z - v2.z});
+    return o;
+}
+
+template <int TX, int TY>
+struct AssembleStShared {
+    static constexpr int WX = TX + 2 * H, WY = TY + 2 * H;
+    static constexpr int NR = TX * TY + strip_max<TX, TY>();
+    float wx[3][WY][WX];
+    float wv[3][WY][WX];
+    float wdv[3][WY][WX];
+    uint32_t wm[WY][WX];
+    float res[12][NR];  // per evaluated pair: f, K w, block term
+    float ws[TX * TY / 64];
+    float bc;
+};
+
+// UPD as in k_assemble.
+template <int TX, int TY, bool UPD>
+__global__ __launch_bounds__(TX* TY, 4) void k_assemble_st(DevCloth c, StepScalars s) {
+    constexpr int NT = TX * TY;
+    using SH = AssembleStShared<TX, TY>;
+    constexpr int WX = SH::WX, WY = SH::WY;
+    constexpr int NWIN = (WX * WY + NT - 1) / NT;
+    static_assert(strip_max<TX, TY>() <= NT, "a strip must fit one round of the block");
+    __shared__ SH sh;
+    const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+    const int tile = tile_of_block(blockIdx.x, c.nt);
+    const int bx = tile % c.ntx, by = tile / c.ntx;
+    const int gi = bx * TX + lx, gj = by * TY + ly;
+    const bool own = gi < c.nx && gj < c.ny;
+    const int N = c.N, nx = c.nx, i = gj * nx + gi;
+    if (blockIdx.x == 0) {
+        if (UPD) cg_finish_block(c, s, &sh.bc);
+        if (threadIdx.x == 0) {
+            c.ctl[0] = 0;
+            c.ctl[1] = 0;
+            c.ctl[2] = 0;
+        }
+    }
+    // the window: x, v, dv and the stencil masks of the tile and its halo
+    const int x0 = bx * TX - H, y0 = by * TY - H;
+#pragma unroll
+    for (int n = 0; n < NWIN; n++) {
+        const int idx = threadIdx.x + n * NT;
+        if (idx < WX * WY) {
+            const int wx = idx % WX, wy = idx / WX;
+            const int gx = x0 + wx, gy = y0 + wy;
+            V3 xw = {0.0f, 0.0f, 0.0f}, vw = xw, dw = xw;
+            uint32_t mw = 0;
+            if (gx >= 0 && gx < nx && gy >= 0 && gy < c.ny) {
+                const int g = gy * nx + gx;
+                mw = ldo(c.asm_mask, (uint32_t)g);
+                xw = ld3o(c.x, N, g);
+                vw = ld3o(c.v, N, g);
+                if (UPD) {
+                    const V3 dp = ld3o(c.dv, N, g);
+                    vw = {vw.x + dp.x, vw.y + dp.y, vw.z + dp.z};
+                    xw = {xw.x + s.h * vw.x, xw.y + s.h * vw.y, xw.z + s.h * vw.z};
+                    if ((s.flags & OC_FLAG_GROUND_CLAMP) && xw.y < 0.0f) xw.y = 0.0f;
+                    dw = dp;
+                } else if (s.warm) {
+                    dw = ld3o(c.dv, N, g);
+                }
+            }
+            sh.wx[0][wy][wx] = xw.x;
+            sh.wx[1][wy][wx] = xw.y;
+            sh.wx[2][wy][wx] = xw.z;
+            sh.wv[0][wy][wx] = vw.x;
+            sh.wv[1][wy][wx] = vw.y;
+            sh.wv[2][wy][wx] = vw.z;
+            sh.wdv[0][wy][wx] = dw.x;
+            sh.wdv[1][wy][wx] = dw.y;
+            sh.wdv[2][wy][wx] = dw.z;
+            sh.wm[wy][wx] = mw;
+        }
+    }
+    // the first direction's constants: this vertex's pair and its strip owner's
+    auto own_cst = [&](int m, int o, int q, int g) {
+        return ldo(c.cst, (uint32_t)(((m * kOwned + o) * 3 + q) * N + g));
+    };
+    float oL0 = 0.0f, oks = 0.0f, okd = 0.0f, sL0 = 0.0f, sks = 0.0f, skd = 0.0f;
+    int swx = 0, swy = 0;
+    bool sin = false;  // this thread has a strip owner on the grid
+    auto prefetch = [&](int o) {
+        if (own) {
+            oL0 = own_cst(0, o, 0, i);
+            oks = own_cst(0, o, 1, i);
+            okd = own_cst(0, o, 2, i);
+        }
+        sin = false;
+        if ((int)threadIdx.x < strip_count<TX, TY>(o)) {
+            strip_owner<TX, TY>(o, threadIdx.x, swx, swy);
+            const int gx = bx * TX + swx, gy = by * TY + swy;
+            if (gx >= 0 && gx < nx && gy >= 0 && gy < c.ny) {
+                const int g = gy * nx + gx;
+                sin = true;
+                sL0 = own_cst(0, o, 0, g);
+                sks = own_cst(0, o, 1, g);
+                skd = own_cst(0, o, 2, g);
+            }
+        }
+    };
+    prefetch(0);
+    __syncthreads();
+
+    const float h = s.h, hh = s.hh;
+    const int cx = lx + H, cy = ly + H;
+    const V3 xi = {sh.wx[0][cy][cx], sh.wx[1][cy][cx], sh.wx[2][cy][cx]};
+    const V3 vi = {sh.wv[0][cy][cx], sh.wv[1][cy][cx], sh.wv[2][cy][cx]};
+    const V3 dvi = {sh.wdv[0][cy][cx], sh.wdv[1][cy][cx], sh.wdv[2][cy][cx]};
+    const uint32_t mi = own ? sh.wm[cy][cx] : 0u;
+    bool pinned = true;
+    V3 acc = {0.0f, 0.0f, 0.0f}, kv = {0.0f, 0.0f, 0.0f};
+    V3 adv = {0.0f, 0.0f, 0.0f}, dg = {0.0f, 0.0f, 0.0f};
+    if (own) {
+        pinned = ldo(c.pin, (uint32_t)i) != 0;
+        if (!s.warm) st3o(c.dv, N, i, dvi);
+        if (UPD) {
+            st3o(c.xo, N, i, xi);
+            st3o(c.vo, N, i, vi);
+            if (!(isfinite(xi.x) && isfinite(xi.y) && isfinite(xi.z) && isfinite(vi.x) &&
+                  isfinite(vi.y) && isfinite(vi.z)))
+                atomicOr(c.ctl + 3, OC_SOLVE_NONFINITE);
+        }
+        if (!pinned) acc = {0.0f + s.mass * s.gx, 0.0f + s.mass * s.gy, 0.0f + s.mass * s.gz};
+        if (!pinned && s.damp != 0.0f)
+            acc = {acc.x + s.damp * vi.x, acc.y + s.damp * vi.y, acc.z + s.damp * vi.z};
+        adv = {s.md * dvi.x, s.md * dvi.y, s.md * dvi.z};
+        dg = {s.md, s.md, s.md};
+    }
+    float C[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    auto window3 = [&](float (&w)[3][WY][WX], int px, int py) {
+        return V3{w[0][py][px], w[1][py][px], w[2][py][px]};
+    };
+    // Add one spring's terms (from this vertex's side) to the pair's sums.
+    auto add_spring = [&](const SpringOut& r) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) C[q] += r.c[q];
+        acc = {acc.x + r.f.x, acc.y + r.f.y, acc.z + r.f.z};
+        kv = {kv.x + r.kw.x, kv.y + r.kw.y, kv.z + r.kw.z};
+    };
+    // The second spring of the pair with the neighbour at window (px, py),
+    // from this vertex's side (constants of the owner g at multiplicity 1).
+    auto add_second = [&](int o, int g, int px, int py) {
+        const SpringOut r = eval_spring(xi, window3(sh.wx, px, py), vi, window3(sh.wv, px, py),
+                                        own_cst(1, o, 0, g), own_cst(1, o, 1, g), own_cst(1, o, 2, g),
+                                        h, hh);
+        add_spring(r);
+    };
+    // The pair is complete: its block (when owned), A dv and diag(A) terms.
+    auto finish_pair = [&](int px, int py, int k_store) {
+        if (k_store >= 0) {
+            const int plane = c.stencil_plane[k_store];
+#pragma unroll
+            for (int q = 0; q < 6; q++)
+                sto(c.blocks + (size_t)q * N, (uint32_t)(plane * 6 * N + i), C[q]);
+        }
+        const V3 dvj = window3(sh.wdv, px, py);
+        const V3 a = symmv(C, V3{dvj.x - dvi.x, dvj.y - dvi.y, dvj.z - dvi.z});
+        adv = {adv.x + a.x, adv.y + a.y, adv.z + a.z};
+        dg = {dg.x - C[0], dg.y - C[1], dg.z - C[2]};
+#pragma unroll
+        for (int q = 0; q < 6; q++) C[q] = 0.0f;
+    };
+
+    // The result stored at LDS column r (the owner's side).
+    auto stored = [&](int r) {
+        SpringOut e;
+        e.f = {sh.res[0][r], sh.res[1][r], sh.res[2][r]};
+        e.kw = {sh.res[3][r], sh.res[4][r], sh.res[5][r]};
+#pragma unroll
+        for (int q = 0; q < 6; q++) e.c[q] = sh.res[6 + q][r];
+        return e;
+    };
+    SpringOut held{};  // this vertex's k = 7 pair, added after k = 5 and k = 6
+#pragma unroll 1
+    for (int o = 0; o < kOwned; o++) {
+        const int dx = own_dx(o), dy = own_dy(o);
+        const int ko = owned_k(o), kp = owned_partner_k(o);
+        // ---- evaluate: this vertex's pair in direction o, then its strip owner's
+        if (own && ((mi >> ko) & 1u)) {
+            const SpringOut e = eval_spring(xi, window3(sh.wx, cx + dx, cy + dy), vi,
+                                            window3(sh.wv, cx + dx, cy + dy), oL0, oks, okd, h, hh);
+            const int r = threadIdx.x;
+            sh.res[0][r] = e.f.x;
+            sh.res[1][r] = e.f.y;
+            sh.res[2][r] = e.f.z;
+            sh.res[3][r] = e.kw.x;
+            sh.res[4][r] = e.kw.y;
+            sh.res[5][r] = e.kw.z;
+#pragma unroll
+            for (int q = 0; q < 6; q++) sh.res[6 + q][r] = e.c[q];
+        }
+        if (sin && ((sh.wm[swy + H][swx + H] >> ko) & 1u)) {
+            const int px = swx + H, py = swy + H;
+            const SpringOut e = eval_spring(window3(sh.wx, px, py), window3(sh.wx, px + dx, py + dy),
+                                            window3(sh.wv, px, py), window3(sh.wv, px + dx, py + dy),
+                                            sL0, sks, skd, h, hh);
+            const int r = NT + threadIdx.x;
+            sh.res[0][r] = e.f.x;
+            sh.res[1][r] = e.f.y;
+            sh.res[2][r] = e.f.z;
+            sh.res[3][r] = e.kw.x;
+            sh.res[4][r] = e.kw.y;
+            sh.res[5][r] = e.kw.z;
+#pragma unroll
+            for (int q = 0; q < 6; q++) sh.res[6 + q][r] = e.c[q];
+        }
+        if (o + 1 < kOwned) prefetch(o + 1);
+        __syncthreads();
+        // ---- accumulate, in this vertex's stencil order
+        if (own && ((mi >> kp) & 1u)) {  // partner of the pair owned at -(dx, dy)
+            const int wx = lx - dx, wy = ly - dy;
+            const int r = (wx >= 0 && wx < TX && wy >= 0 && wy < TY) ? wy * TX + wx
+                                                                       : NT + strip_index<TX, TY>(o, wx, wy);
+            SpringOut e = stored(r);
+            e.f = {0.0f - e.f.x, 0.0f - e.f.y, 0.0f - e.f.z};
+            e.kw = {0.0f - e.kw.x, 0.0f - e.kw.y, 0.0f - e.kw.z};
+            add_spring(e);
+            if ((mi >> (16 + kp)) & 1u) add_second(o, i - dy * nx - dx, cx - dx, cy - dy);
+            finish_pair(cx - dx, cy - dy, -1);
+        }
+        if (o == 2) {
+            if (own && ((mi >> ko) & 1u)) held = stored(threadIdx.x);  // k = 7 follows k = 5, 6
+        } else if (own) {
+#pragma unroll
+            for (int t = 0; t < 2; t++) {  // k = 6, then the held k = 7; else just ko
+                const int oo = o == 3 ? (t == 0 ? 3 : 2) : o;
+                if (t == 1 && o != 3) break;
+                const int kk = owned_k(oo), ddx = own_dx(oo), ddy = own_dy(oo);
+                if (!((mi >> kk) & 1u)) continue;
+                add_spring(t == 0 ? stored(threadIdx.x) : held);
+                if ((mi >> (16 + kk)) & 1u) add_second(oo, i, cx + ddx, cy + ddy);
+                finish_pair(cx + ddx, cy + ddy, kk);
+            }
+        }
+        __syncthreads();
+    }
+
+    float t = 0.0f;
+    if (own) {
         V3 b;
         b.x = pinned ? 0.0f : h * (acc.x + h * kv.x);
         b.y = pinned ? 0.0f : h * (acc.y + h * kv.y);
@@ -439,7 +739,7 @@ __device__ inline bool spmv_phase(const DevCloth& c, const StepScalars& s, int k
     float t = 0.0f;
     if (own) {
         const V3 di = {sh.wd[0][ly + H][lx + H], sh.wd[1][ly + H][lx + H], sh.wd[2][ly + H][lx + H]};
-        V3 acc = {s.mass * di.x, s.mass * di.y, s.mass * di.z};
+        V3 acc = {s.md * di.x, s.md * di.y, s.md * di.z};
         if constexpr (STENCIL) {
 #pragma unroll
             for (int k = 0; k < kStencil; k++) {
@@ -472,6 +772,256 @@ template <int TX, int TY, int WP, bool STENCIL>
 __global__ __launch_bounds__(TX* TY) void k_cg_spmv(DevCloth c, StepScalars s, int k) {
     __shared__ SpmvShared<TX, TY> sh;
     spmv_phase<TX, TY, WP, STENCIL>(c, s, k, true, sh, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// single-reduction CG (OC_FLAG_CG_SINGLE_REDUCTION): one launch per iteration
+// ---------------------------------------------------------------------------
+// The Chronopoulos-Gear recurrence (s = 1): with u = P^-1 r and w = S A u the
+// two dots gamma = r.u and delta = w.u of an iteration come from the same
+// launch, so each iteration needs one reduction point, and one launch:
+//   beta = gamma_k / gamma_{k-1},  alpha = gamma_k / (delta_k - beta gamma_k / alpha_{k-1})
+//   p = u + beta p,  s = w + beta s  (s = S A p),  dv += alpha p,  r -= alpha s,
+//   u = P^-1 r,  w = S A u,  partials of r.u and w.u.
+// r, s and w are read on the tile's halo by neighbouring blocks while the
+// launch writes their next values, so each has two buffers used by iteration
+// parity; p and dv are only read on the tile itself.  The start (w0 = S A u0,
+// delta_0) is the first launch of the standard SpMV, whose q is w0.  The
+// arithmetic and guards are the oracle's cg_core_cg1, step for step.
+template <int TX, int TY>
+struct Cg1Shared {
+    static constexpr int WX = TX + 2 * H, WY = TY + 2 * H;
+    float wu[3][WY][WX];  // u_{k+1} on the tile and halo
+    float rt[3][TY][TX];  // r_{k+1} of the tile
+    float ws[TX * TY / 64];
+    float ws2[TX * TY / 64];
+    float bc, bc2;
+};
+
+// Buffers of iteration k: r_k, w_k, s_{k-1} are read, r_{k+1}, w_{k+1}, s_k
+// written (parity k & 1).
+struct Cg1Bufs {
+    const float* r_in;
+    float* r_out;
+    const float* w_in;
+    float* w_out;
+    const float* s_in;
+    float* s_out;
+};
+__device__ inline Cg1Bufs cg1_bufs(const DevCloth& c, int k) {
+    Cg1Bufs b;
+    const bool odd = (k & 1) != 0;
+    b.r_in = odd ? c.r2 : c.r;
+    b.r_out = odd ? c.r : c.r2;
+    b.w_in = odd ? c.q2 : c.q;
+    b.w_out = odd ? c.q : c.q2;
+    b.s_in = odd ? c.s1 : c.s2;
+    b.s_out = odd ? c.s2 : c.s1;
+    return b;
+}
+
+template <int TX, int TY, int WP, bool STENCIL>
+__global__ __launch_bounds__(TX* TY) void k_cg1(DevCloth c, StepScalars s, int k) {
+    constexpr int NT = TX * TY;
+    constexpr int WX = TX + 2 * H, WY = TY + 2 * H;
+    constexpr int NWIN = (WX * WY + NT - 1) / NT;
+    constexpr int NP = 8;  // partials per lane loaded up front, per dot
+    __shared__ Cg1Shared<TX, TY> sh;
+    const int nt = c.nt, N = c.N;
+    const int tile = tile_of_block(blockIdx.x, nt);
+    const int bx = tile % c.ntx, by = tile / c.ntx;
+    const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+    const int gi = bx * TX + lx, gj = by * TY + ly;
+    const bool own = gi < c.nx && gj < c.ny;
+    const int i = gj * c.nx + gi;
+    const bool jac = (s.flags & OC_FLAG_JACOBI) != 0;
+    const Cg1Bufs bf = cg1_bufs(c, k);
+
+    // ---- loads that do not depend on the reduction
+    const int done = c.ctl[0];
+    const float gamma_old = k > 0 ? c.scal[k - 1] : 0.0f;
+    const float alpha_old = k > 0 ? c.scal[c.cg1_alpha + k - 1] : 0.0f;
+    const float gamma0 = c.scal[0];
+    float pg[NP], pd[NP];
+    if (threadIdx.x < 64) {
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            const int t = threadIdx.x + 64 * j;
+            pg[j] = t < nt ? c.del_part[(size_t)k * nt + t] : 0.0f;
+            pd[j] = t < nt ? c.sig_part[(size_t)k * nt + t] : 0.0f;
+        }
+    }
+    const int x0 = bx * TX - H, y0 = by * TY - H;
+    V3 wr[NWIN], ww[NWIN], wsv[NWIN], wpv[NWIN];
+#pragma unroll
+    for (int n = 0; n < NWIN; n++) {
+        const int idx = threadIdx.x + n * NT;
+        const int gx = x0 + idx % WX, gy = y0 + idx / WX;
+        wr[n] = ww[n] = wsv[n] = wpv[n] = V3{0.0f, 0.0f, 0.0f};
+        if (idx < WX * WY && gx >= 0 && gx < c.nx && gy >= 0 && gy < c.ny) {
+            const int g = gy * c.nx + gx;
+            wr[n] = ld3o(bf.r_in, N, g);
+            ww[n] = ld3o(bf.w_in, N, g);
+            if (k > 0) wsv[n] = ld3o(bf.s_in, N, g);
+            if (jac) wpv[n] = ld3o(c.pinv, N, g);
+        }
+    }
+    V3 pold = {0.0f, 0.0f, 0.0f}, x = pold, rk = pold, pv = pold;
+    if (own) {
+        if (k > 0) pold = ld3o(c.d1, N, i);
+        x = ld3o(c.dv, N, i);
+        rk = ld3o(bf.r_in, N, i);
+        if (jac) pv = ld3o(c.pinv, N, i);
+    }
+    uint32_t mask = 0;
+    if constexpr (STENCIL) mask = own ? ldo(c.pair_mask, (uint32_t)i) : 0u;
+    const bool pinned = own ? ldo(c.pin, (uint32_t)i) != 0 : true;
+
+    // ---- gamma_k, delta_k (the same lane tree as the other launches)
+    if (threadIdx.x < 64) {
+        float sg = 0.0f, sd = 0.0f;
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            sg += pg[j];
+            sd += pd[j];
+        }
+        for (int t = threadIdx.x + 64 * NP; t < nt; t += 64) {
+            sg += c.del_part[(size_t)k * nt + t];
+            sd += c.sig_part[(size_t)k * nt + t];
+        }
+        sg = wave_sum(sg);
+        sd = wave_sum(sd);
+        if (threadIdx.x == 0) {
+            sh.bc = sg;
+            sh.bc2 = sd;
+        }
+    }
+    __syncthreads();
+    if (done) return;
+    const float gamma = sh.bc, delta = sh.bc2;
+    const float thresh = s.tol_mode == 1 ? s.tol2 * (k == 0 ? gamma : gamma0) : s.tol2;
+    const bool stop = s.tol_mode != 2 && !(gamma > thresh);
+    const float beta = (k > 0 && gamma_old != 0.0f) ? gamma / gamma_old : 0.0f;
+    float den = delta;
+    if (beta != 0.0f && alpha_old != 0.0f) {
+        float t = beta * gamma;
+        t = t / alpha_old;
+        den = delta - t;
+    }
+    const bool brk = !stop && (den < 0.0f || !isfinite(den));
+    const float alpha = den == 0.0f ? 0.0f : gamma / den;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c.scal[k] = gamma;
+        c.scal[c.cg1_alpha + k] = alpha;
+        if (stop || brk) {
+            c.ctl[0] = 1;
+            c.ctl[1] = k;
+            if (brk) c.ctl[2] |= OC_SOLVE_BREAKDOWN;
+        }
+    }
+    if (stop || brk) return;
+
+    // ---- pair blocks of this vertex (the SpMV below)
+    constexpr int NS = STENCIL ? kStencil : WP;
+    uint16_t m[NS];
+    float C[NS][6];
+    if constexpr (STENCIL) {
+#pragma unroll
+        for (int q = 0; q < kStencil; q++) {
+            const int di = stencil_di(q), dj = stencil_dj(q);
+            const uint32_t blk = stencil_owner(q) ? (uint32_t)i : (uint32_t)(i + dj * c.nx + di);
+            const uint32_t off = blk + (uint32_t)c.stencil_plane[q] * 6u * (uint32_t)N;
+#pragma unroll
+            for (int e = 0; e < 6; e++) C[q][e] = ((mask >> q) & 1u) ? ldo(c.blocks, off + (uint32_t)(e * N)) : 0.0f;
+        }
+    } else {
+#pragma unroll
+        for (int sl = 0; sl < WP; sl++) {
+            m[sl] = (own && sl < c.wp) ? ldo(c.pair_m, (uint32_t)(sl * N + i)) : kEmptySlot;
+            if (m[sl] != kEmptySlot) {
+                const int j = i + slot_dj(m[sl]) * c.nx + slot_di(m[sl]);
+                const int blk = ((m[sl] >> 12) & 1) ? i : j;
+                const uint32_t b0 = (uint32_t)((m[sl] >> 8) & 15) * 6u * (uint32_t)N + (uint32_t)blk;
+#pragma unroll
+                for (int e = 0; e < 6; e++) C[sl][e] = ldo(c.blocks + (size_t)e * N, b0);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 6; e++) C[sl][e] = 0.0f;
+            }
+        }
+    }
+
+    // ---- s_k, r_{k+1}, u_{k+1} on the tile + halo; the tile's own values out
+#pragma unroll
+    for (int n = 0; n < NWIN; n++) {
+        const int idx = threadIdx.x + n * NT;
+        if (idx < WX * WY) {
+            const int wx = idx % WX, wy = idx / WX;
+            const V3 sk = k == 0 ? ww[n] : V3{ww[n].x + beta * wsv[n].x, ww[n].y + beta * wsv[n].y,
+                                              ww[n].z + beta * wsv[n].z};
+            const V3 rn = {wr[n].x - alpha * sk.x, wr[n].y - alpha * sk.y, wr[n].z - alpha * sk.z};
+            const V3 un = jac ? V3{wpv[n].x * rn.x, wpv[n].y * rn.y, wpv[n].z * rn.z} : rn;
+            const int gx = x0 + wx, gy = y0 + wy;
+            if (wx >= H && wx < H + TX && wy >= H && wy < H + TY && gx < c.nx && gy < c.ny) {
+                const int g = gy * c.nx + gx;
+                st3o(bf.s_out, N, g, sk);
+                st3o(bf.r_out, N, g, rn);
+            }
+            if (wx >= H && wx < H + TX && wy >= H && wy < H + TY) {
+                sh.rt[0][wy - H][wx - H] = rn.x;
+                sh.rt[1][wy - H][wx - H] = rn.y;
+                sh.rt[2][wy - H][wx - H] = rn.z;
+            }
+            sh.wu[0][wy][wx] = un.x;
+            sh.wu[1][wy][wx] = un.y;
+            sh.wu[2][wy][wx] = un.z;
+        }
+    }
+    // this vertex: p_k = u_k + beta p_{k-1}, dv += alpha p_k
+    if (own) {
+        const V3 uk = jac ? V3{pv.x * rk.x, pv.y * rk.y, pv.z * rk.z} : rk;
+        const V3 pk = k == 0 ? uk : V3{uk.x + beta * pold.x, uk.y + beta * pold.y, uk.z + beta * pold.z};
+        st3o(c.d1, N, i, pk);
+        st3o(c.dv, N, i, V3{x.x + alpha * pk.x, x.y + alpha * pk.y, x.z + alpha * pk.z});
+    }
+    __syncthreads();
+
+    // ---- w_{k+1} = S A u_{k+1}; partials of r.u and w.u
+    float tg = 0.0f, td = 0.0f;
+    if (own) {
+        const V3 ui = {sh.wu[0][ly + H][lx + H], sh.wu[1][ly + H][lx + H], sh.wu[2][ly + H][lx + H]};
+        V3 acc = {s.md * ui.x, s.md * ui.y, s.md * ui.z};
+        if constexpr (STENCIL) {
+#pragma unroll
+            for (int q = 0; q < kStencil; q++) {
+                if (!((mask >> q) & 1u)) continue;
+                const int wy = ly + H + stencil_dj(q), wx = lx + H + stencil_di(q);
+                V3 w = {sh.wu[0][wy][wx] - ui.x, sh.wu[1][wy][wx] - ui.y, sh.wu[2][wy][wx] - ui.z};
+                V3 a = symmv(C[q], w);
+                acc = {acc.x + a.x, acc.y + a.y, acc.z + a.z};
+            }
+        } else {
+#pragma unroll
+            for (int sl = 0; sl < WP; sl++) {
+                if (m[sl] == kEmptySlot) break;
+                const int wy = ly + H + slot_dj(m[sl]), wx = lx + H + slot_di(m[sl]);
+                V3 w = {sh.wu[0][wy][wx] - ui.x, sh.wu[1][wy][wx] - ui.y, sh.wu[2][wy][wx] - ui.z};
+                V3 a = symmv(C[sl], w);
+                acc = {acc.x + a.x, acc.y + a.y, acc.z + a.z};
+            }
+        }
+        const V3 wn = pinned ? V3{0.0f, 0.0f, 0.0f} : acc;
+        st3o(bf.w_out, N, i, wn);
+        const V3 rn = {sh.rt[0][ly][lx], sh.rt[1][ly][lx], sh.rt[2][ly][lx]};
+        tg = dot3(rn, ui);
+        td = dot3(wn, ui);
+    }
+    const float pgs = tile_sum<NT>(tg, sh.ws);
+    const float pds = tile_sum<NT>(td, sh.ws2);
+    if (threadIdx.x == 0) {
+        c.del_part[(size_t)(k + 1) * nt + tile] = pgs;
+        c.sig_part[(size_t)(k + 1) * nt + tile] = pds;
+    }
 }
 
 // CG iteration k, part 2: dv += alpha d, r -= alpha q, partial r.r.  Returns
@@ -690,6 +1240,8 @@ __global__ __launch_bounds__(TX* TY) void k_adj_gather(DevCloth c, StepScalars s
     const V3 xi = ld3(c.x, N, i), vi = ld3(c.v, N, i), mi = ld3(a.mu, N, i), pi = ld3(c.dv, N, i);
     const V3 wi = add3(vi, pi);
     V3 gx = ld3(a.gx_out, N, i), gv = ld3(a.gvt, N, i);
+    if (s.damp != 0.0f && !c.pin[i])  // the damping force c v_i: dJ/dv_i += h c mu_i
+        gv = {gv.x + s.h * (s.damp * mi.x), gv.y + s.h * (s.damp * mi.y), gv.z + s.h * (s.damp * mi.z)};
     for (int k = 0; k < c.wa; k++) {
         const int sp = c.ainc_s[k * N + i];
         if (sp < 0) break;
@@ -718,7 +1270,26 @@ __global__ __launch_bounds__(TX* TY) void k_adj_gather(DevCloth c, StepScalars s
     st3(a.gv_in, N, i, gv);
 }
 
+thread_local LaunchProfile* t_prof = nullptr;
+
+// Launch with the installed profile's next event pair, if any (see
+// LaunchProfile).
+template <typename F, typename... Args>
+hipError_t launch_k(int kind, F kernel, dim3 grid, dim3 block, hipStream_t st, Args... args) {
+    LaunchProfile* p = t_prof;
+    if (p && p->n < p->cap) {
+        const int k = p->n++;
+        p->kind[k] = kind;
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, st, p->start[k], p->stop[k], 0, args...);
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+    }
+    return hipGetLastError();
+}
+
 }  // namespace
+
+void set_launch_profile(LaunchProfile* p) { t_prof = p; }
 
 hipError_t launch_adj_init(const DevCloth& c, const StepScalars& s, const AdjointBufs& a,
                            hipStream_t st) {
@@ -746,44 +1317,50 @@ hipError_t launch_assemble(const DevCloth& c, const StepScalars& s, hipStream_t 
     // solve without iterations leaves where this assembly writes its own).
     if (fused_update && (!s.warm || s.max_it < 1)) return hipErrorInvalidValue;
     const dim3 grid(c.nt), block(kTileX * kTileY);
-    if (c.wa <= 16) {
-        if (fused_update)
-            hipLaunchKernelGGL((k_assemble<kTileX, kTileY, 16, true>), grid, block, 0, st, c, s);
-        else
-            hipLaunchKernelGGL((k_assemble<kTileX, kTileY, 16, false>), grid, block, 0, st, c, s);
-    } else {
-        if (fused_update)
-            hipLaunchKernelGGL((k_assemble<kTileX, kTileY, kMaxAssemblyWidth, true>), grid, block, 0, st,
-                               c, s);
-        else
-            hipLaunchKernelGGL((k_assemble<kTileX, kTileY, kMaxAssemblyWidth, false>), grid, block, 0,
-                               st, c, s);
+    if (c.stencil_asm)
+        return fused_update ? launch_k(0, k_assemble_st<kTileX, kTileY, true>, grid, block, st, c, s)
+                            : launch_k(0, k_assemble_st<kTileX, kTileY, false>, grid, block, st, c, s);
+    if (c.wa <= 16)
+        return fused_update ? launch_k(0, k_assemble<kTileX, kTileY, 16, true>, grid, block, st, c, s)
+                            : launch_k(0, k_assemble<kTileX, kTileY, 16, false>, grid, block, st, c, s);
+    return fused_update
+               ? launch_k(0, k_assemble<kTileX, kTileY, kMaxAssemblyWidth, true>, grid, block, st, c, s)
+               : launch_k(0, k_assemble<kTileX, kTileY, kMaxAssemblyWidth, false>, grid, block, st, c, s);
+}
+
+// cst[spring_cst[sp] + q*N] = (L0, ks, kd)[q][sp] for q in [q0, q0 + nq).
+__global__ __launch_bounds__(256) void k_scatter_cst(DevCloth c, const int32_t* spring_cst, int S, int q0,
+                                                     int nq) {
+    const float* src[3] = {c.L0, c.ks, c.kd};
+    float* dst = const_cast<float*>(c.cst);
+    for (int sp = blockIdx.x * 256 + threadIdx.x; sp < S; sp += gridDim.x * 256) {
+        const size_t base = (size_t)spring_cst[sp];
+        for (int q = q0; q < q0 + nq; q++) dst[base + (size_t)q * c.N] = src[q][sp];
     }
+}
+
+hipError_t launch_scatter_cst(const DevCloth& c, const int32_t* spring_cst, int S, int q0, int nq,
+                              hipStream_t st) {
+    if (q0 < 0 || nq < 0 || q0 + nq > 3) return hipErrorInvalidValue;
+    if (S <= 0 || nq == 0) return hipSuccess;
+    const int grid = std::min((S + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_scatter_cst, dim3(grid), dim3(256), 0, st, c, spring_cst, S, q0, nq);
     return hipGetLastError();
 }
 
 hipError_t launch_cg_spmv(const DevCloth& c, const StepScalars& s, int k, hipStream_t st) {
-    if (c.stencil)
-        hipLaunchKernelGGL((k_cg_spmv<kTileX, kTileY, kStencil, true>), dim3(c.nt),
-                           dim3(kTileX * kTileY), 0, st, c, s, k);
-    else if (c.wp <= 12)
-        hipLaunchKernelGGL((k_cg_spmv<kTileX, kTileY, 12, false>), dim3(c.nt), dim3(kTileX * kTileY),
-                           0, st, c, s, k);
-    else
-        hipLaunchKernelGGL((k_cg_spmv<kTileX, kTileY, kMaxPairWidth, false>), dim3(c.nt),
-                           dim3(kTileX * kTileY), 0, st, c, s, k);
-    return hipGetLastError();
+    const dim3 grid(c.nt), block(kTileX * kTileY);
+    if (c.stencil) return launch_k(1, k_cg_spmv<kTileX, kTileY, kStencil, true>, grid, block, st, c, s, k);
+    if (c.wp <= 12) return launch_k(1, k_cg_spmv<kTileX, kTileY, 12, false>, grid, block, st, c, s, k);
+    return launch_k(1, k_cg_spmv<kTileX, kTileY, kMaxPairWidth, false>, grid, block, st, c, s, k);
 }
 
 hipError_t launch_cg_axpy(const DevCloth& c, const StepScalars& s, int k, hipStream_t st) {
-    hipLaunchKernelGGL((k_cg_axpy<kTileX, kTileY>), dim3(c.nt), dim3(kTileX * kTileY), 0, st, c,
-                       s, k);
-    return hipGetLastError();
+    return launch_k(2, k_cg_axpy<kTileX, kTileY>, dim3(c.nt), dim3(kTileX * kTileY), st, c, s, k);
 }
 
 hipError_t launch_cg_finish(const DevCloth& c, const StepScalars& s, hipStream_t st) {
-    hipLaunchKernelGGL(k_cg_finish, dim3(1), dim3(64), 0, st, c, s);
-    return hipGetLastError();
+    return launch_k(3, k_cg_finish, dim3(1), dim3(64), st, c, s);
 }
 
 int persistent_cg_grid(const DevCloth& c) {
@@ -811,7 +1388,7 @@ int persistent_cg_grid(const DevCloth& c) {
 
 hipError_t launch_cg(const DevCloth& c, const StepScalars& s, hipStream_t st, bool finish) {
     hipError_t e;
-    if (c.persistent && s.max_it > 0) {
+    if (c.persistent && s.max_it > 0 && !(s.flags & OC_FLAG_CG_SINGLE_REDUCTION)) {
         if (c.persistent > c.nt) return hipErrorInvalidValue;  // a block without a tile
         unsigned* bar = reinterpret_cast<unsigned*>(c.ctl + 6);
         hipLaunchKernelGGL(k_reset_barrier, dim3(1), dim3(64), 0, st, bar);
@@ -825,6 +1402,22 @@ hipError_t launch_cg(const DevCloth& c, const StepScalars& s, hipStream_t st, bo
         if (e != hipSuccess) return e;
         return finish ? launch_cg_finish(c, s, st) : hipSuccess;
     }
+    if (s.flags & OC_FLAG_CG_SINGLE_REDUCTION) {
+        // the start: w0 = S A u0 and delta_0 (the first standard SpMV), then
+        // one launch per iteration
+        if (s.max_it > 0 && (e = launch_cg_spmv(c, s, 0, st)) != hipSuccess) return e;
+        const dim3 grid(c.nt), block(kTileX * kTileY);
+        for (int k = 0; k < s.max_it; k++) {
+            if (c.stencil)
+                e = launch_k(1, k_cg1<kTileX, kTileY, kStencil, true>, grid, block, st, c, s, k);
+            else if (c.wp <= 12)
+                e = launch_k(1, k_cg1<kTileX, kTileY, 12, false>, grid, block, st, c, s, k);
+            else
+                e = launch_k(1, k_cg1<kTileX, kTileY, kMaxPairWidth, false>, grid, block, st, c, s, k);
+            if (e != hipSuccess) return e;
+        }
+        return finish ? launch_cg_finish(c, s, st) : hipSuccess;
+    }
     for (int k = 0; k < s.max_it; k++) {
         if ((e = launch_cg_spmv(c, s, k, st)) != hipSuccess) return e;
         if ((e = launch_cg_axpy(c, s, k, st)) != hipSuccess) return e;
@@ -833,11 +1426,9 @@ hipError_t launch_cg(const DevCloth& c, const StepScalars& s, hipStream_t st, bo
 }
 
 hipError_t launch_update(const DevCloth& c, const StepScalars& s, hipStream_t st, bool finish) {
-    if (finish)
-        hipLaunchKernelGGL(k_update<true>, dim3((c.N + 255) / 256), dim3(256), 0, st, c, s);
-    else
-        hipLaunchKernelGGL(k_update<false>, dim3((c.N + 255) / 256), dim3(256), 0, st, c, s);
-    return hipGetLastError();
+    const dim3 grid((c.N + 255) / 256), block(256);
+    return finish ? launch_k(4, k_update<true>, grid, block, st, c, s)
+                  : launch_k(4, k_update<false>, grid, block, st, c, s);
 }
 
 }  // namespace oc

@@ -27,8 +27,8 @@
 namespace {
 
 // 4 MiB: the CG partials and scalars of one solve take
-// tiles x (2 cg_max_iters + 1) + cg_max_iters + 2 floats, e.g. 31,264 for a
-// 512x512 cloth (512 tiles) at 30 iterations.
+// 2 (tiles + 1)(cg_max_iters + 1) + 2 floats, e.g. 31,808 for a 512x512
+// cloth (512 tiles) at 30 iterations.
 constexpr size_t kScratchFloats = size_t(1) << 20;
 
 static __device__ __align__(256) float scratch[kScratchFloats];
@@ -62,7 +62,7 @@ extern "C" int oc_cloth_step_noalloc(oc_cloth cloth, int32_t nsteps) {
                              "the CG partials of this cloth need " + std::to_string(need) +
                                  " floats, more than the fixed scratch array holds (" +
                                  std::to_string(kScratchFloats) +
-                                 "): tiles x (2 cg_max_iters + 1) + cg_max_iters + 2 must fit");
+                                 "): 2 (tiles + 1)(cg_max_iters + 1) + 2 must fit");
     int dev = 0;
     OC_HIP(hipGetDevice(&dev));
     if (dev != oc::cloth_device(cloth))

@@ -53,6 +53,69 @@ __host__ __device__ constexpr bool stencil_owner(int k) {
     return stencil_dj(k) > 0 || (stencil_dj(k) == 0 && stencil_di(k) > 0);
 }
 
+// The six stencil positions whose pair the vertex owns (its neighbour has
+// the larger index), in the order the spring-once assembly visits them:
+// +1 (k 1), +nx (k 3), +nx+1 (k 7), +nx-1 (k 6), +2 (k 9), +2nx (k 11).  The
+// partner's position of each is kOwnedPartner (the opposite offset).
+constexpr int kOwned = 6;
+__host__ __device__ constexpr int owned_k(int o) {
+    return o == 0 ? 1 : o == 1 ? 3 : o == 2 ? 7 : o == 3 ? 6 : o == 4 ? 9 : 11;
+}
+__host__ __device__ constexpr int owned_partner_k(int o) {
+    return o == 0 ? 0 : o == 1 ? 2 : o == 2 ? 4 : o == 3 ? 5 : o == 4 ? 8 : 10;
+}
+__host__ __device__ constexpr int own_dx(int o) { return stencil_di(owned_k(o)); }
+__host__ __device__ constexpr int own_dy(int o) { return stencil_dj(owned_k(o)); }
+
+// The strip of owned direction o around a TX x TY tile: owners outside the
+// tile whose partner (owner + (dx, dy)) is inside, in tile coordinates.
+// dx > 0, dy == 0: the dx columns left of the tile; dy > 0, dx == 0: the dy
+// rows above it; diagonals: the row above (TX owners) and the column on the
+// side the diagonal comes from (TY - 1 owners).
+template <int TX, int TY>
+__host__ __device__ inline int strip_count(int o) {
+    const int dx = own_dx(o), dy = own_dy(o);
+    return dy == 0 ? dx * TY : dx == 0 ? dy * TX : TX + TY - 1;
+}
+template <int TX, int TY>
+__host__ __device__ constexpr int strip_max() {
+    return (2 * TX > 2 * TY ? 2 * TX : 2 * TY) > TX + TY - 1 ? (2 * TX > 2 * TY ? 2 * TX : 2 * TY)
+                                                             : TX + TY - 1;
+}
+// Strip index of the owner at tile coordinates (wx, wy) (outside the tile,
+// its partner inside).
+template <int TX, int TY>
+__host__ __device__ inline int strip_index(int o, int wx, int wy) {
+    const int dx = own_dx(o), dy = own_dy(o);
+    if (dy == 0) return (wx + dx) * TY + wy;
+    if (dx == 0) return (wy + dy) * TX + wx;
+    if (wy == -1) return dx > 0 ? wx + 1 : wx - 1;
+    return TX + wy;
+}
+// Owner of strip index s.
+template <int TX, int TY>
+__host__ __device__ inline void strip_owner(int o, int s, int& wx, int& wy) {
+    const int dx = own_dx(o), dy = own_dy(o);
+    if (dy == 0) {
+        wx = s / TY - dx;
+        wy = s % TY;
+    } else if (dx == 0) {
+        wx = s % TX;
+        wy = s / TX - dy;
+    } else if (s < TX) {
+        wx = dx > 0 ? s - 1 : s + 1;
+        wy = -1;
+    } else {
+        wx = dx > 0 ? -1 : TX;
+        wy = s - TX;
+    }
+}
+
+// Constants of the owner-plane copy: (multiplicity m, owned direction o,
+// constant q = L0, ks, kd) of the pair owned by vertex i at
+// cst[((m * kOwned + o) * 3 + q) * N + i].
+constexpr int kCstPlanes = 2 * kOwned * 3;
+
 // Grid topology in the device layout, built on the host from a spring list.
 struct GridTopology {
     int nx = 0, ny = 0, N = 0, S = 0, P = 0;
@@ -66,6 +129,14 @@ struct GridTopology {
     bool stencil = false;                          // pairs follow the kStencil order
     std::vector<uint16_t> pair_mask;               // N, bit k: pair at stencil position k
     int stencil_plane[kStencil] = {};              // plane of the pair at position k
+    // stencil assembly form: the stencil form with at most two springs per
+    // pair.  asm_mask: bit k pair at stencil position k, bit 16 + k two
+    // springs there.  spring_cst: per spring, the index of its L0 in the
+    // owner-plane constants (kOwnedPlanes planes of N per multiplicity, see
+    // owned_slot), ks and kd follow at +N and +2N.
+    bool stencil_asm = false;
+    std::vector<uint32_t> asm_mask;
+    std::vector<int32_t> spring_cst;
     std::vector<int32_t> pair_a, pair_b, pair_plane;
     std::vector<int32_t> spring_pair;
 };
@@ -114,6 +185,17 @@ struct DevCloth {
     // fused update (assembly only): x + h(v + dv) and v + dv are written here
     float* xo;
     float* vo;
+    // stencil assembly form (each spring evaluated once per tile)
+    int stencil_asm;
+    const uint32_t* asm_mask;  // per vertex: bit k pair at position k, bit 16 + k two springs
+    const float* cst;          // owner-plane constants, kCstPlanes * N
+    // single-reduction CG: second buffers of r and w (q), the two s buffers,
+    // and the offset of the alpha values in scal
+    float* r2;
+    float* q2;
+    float* s1;
+    float* s2;
+    int cg1_alpha;
 };
 
 // Implicit steps of a cloth handle, the body of oc_cloth_step (capi.cpp).
@@ -129,7 +211,24 @@ int persistent_cg_grid(const DevCloth& c);
 struct StepScalars {
     float mass, gx, gy, gz, h, hh, tol2;
     int max_it, tol_mode, warm, flags;
+    float damp;  // velocity damping c (force c v on free vertices)
+    float md;    // diagonal mass of A: mass - h c (mass when c = 0)
 };
+
+// Per-launch timing of the implicit step's kernels (oc_cloth_profile_steps).
+// While a LaunchProfile is installed on the calling thread, each launcher of
+// implicit.hip starts its kernel with hipExtLaunchKernelGGL and the next
+// start/stop event pair of the profile, so every kernel's own duration is
+// recorded inside the real launch sequence, with no event packets between
+// the launches.  kind: 0 assembly, 1 CG iteration (part 1), 2 CG part 2,
+// 3 other solver launches, 4 update.
+struct LaunchProfile {
+    hipEvent_t* start;
+    hipEvent_t* stop;
+    int* kind;
+    int n, cap;
+};
+void set_launch_profile(LaunchProfile* p);  // NULL: off
 
 // Kernel launchers (implicit.hip).  All return hipError_t of the launch.
 hipError_t launch_forces(const DevCloth& c, const StepScalars& s, hipStream_t st);
@@ -139,6 +238,10 @@ hipError_t launch_forces(const DevCloth& c, const StepScalars& s, hipStream_t st
 // assembly).
 hipError_t launch_assemble(const DevCloth& c, const StepScalars& s, hipStream_t st,
                            bool fused_update = false);
+// Fill the owner-plane constants (stencil assembly form) from the per-spring
+// L0, ks, kd: constants q in [q0, q0 + nq) (0 L0, 1 ks, 2 kd).
+hipError_t launch_scatter_cst(const DevCloth& c, const int32_t* spring_cst, int S, int q0, int nq,
+                              hipStream_t st);
 // finish = false leaves the iteration count and status to launch_update(...,
 // finish = true), the step's last launch.
 hipError_t launch_cg(const DevCloth& c, const StepScalars& s, hipStream_t st, bool finish = true);
@@ -230,6 +333,11 @@ struct DevPbd {
     const float* b_kp;
     float* part;       // kPbdSums * nblk block partials
     float* sums;       // kPbdSums reduced values
+    // list order (OC_FLAG_LIST_ORDER): the constraints are stored level by
+    // level; level offsets (levels + 1 each) on the device
+    const int32_t* d_lv;
+    const int32_t* b_lv;
+    int nd_lv, nb_lv;
 };
 struct PbdScalars {
     float mass, gx, gy, gz, dt, gd, kdamp, global_k, inv_dt;
@@ -240,6 +348,14 @@ hipError_t launch_pbd_distance(const DevPbd& c, int first, int count, hipStream_
 hipError_t launch_pbd_bending(const DevPbd& c, const PbdScalars& s, int first, int count,
                               hipStream_t st);
 hipError_t launch_pbd_ground(const DevPbd& c, hipStream_t st);
+// The prediction with sequential centre-of-mass sums (list order): one
+// workgroup, the reference's summation order.
+hipError_t launch_pbd_predict_seq(const DevPbd& c, const PbdScalars& s, hipStream_t st);
+// All solver iterations of the list-order schedule in one workgroup: per
+// iteration the distance levels, the bending levels and (ground != 0) the
+// ground clamp, with a workgroup barrier after each.
+hipError_t launch_pbd_levels(const DevPbd& c, const PbdScalars& s, int iterations, int ground,
+                             hipStream_t st);
 hipError_t launch_pbd_integrate(const DevPbd& c, const PbdScalars& s, hipStream_t st);
 
 // Tile shape of the implicit kernels (one workgroup per tile; a multiple of

@@ -36,6 +36,18 @@ __device__ inline V3 cross3(V3 x, V3 y) {
     return {x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y};
 }
 
+// Velocity update of particle i (gravity, global damping, V += (F dt) W;
+// PBD:601-631); returns the updated velocity.
+__device__ inline V3 external_velocity(const DevPbd& c, const PbdScalars& s, int i) {
+    const float wi = c.w[i];
+    V3 v = ld3(c.v, c.N, i);
+    const V3 F = wi > 0.0f ? V3{0.0f + s.gx, 0.0f + s.gy, 0.0f + s.gz} : V3{0.0f, 0.0f, 0.0f};
+    v = {v.x * s.gd, v.y * s.gd, v.z * s.gd};
+    v = {v.x + (F.x * s.dt) * wi, v.y + (F.y * s.dt) * wi, v.z + (F.z * s.dt) * wi};
+    st3(c.v, c.N, i, v);
+    return v;
+}
+
 // Block partials of Q values per thread: wave trees, waves in order.
 template <int Q>
 __device__ inline void block_partials(const float (&t)[Q], float* part, int nblk, int first) {
@@ -60,13 +72,8 @@ __global__ __launch_bounds__(NB) void k_pbd_pass_a(DevPbd c, PbdScalars s) {
     const int i = blockIdx.x * NB + threadIdx.x;
     float t[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     if (i < c.N) {
-        const float wi = c.w[i];
         const V3 x = ld3(c.x, c.N, i);
-        V3 v = ld3(c.v, c.N, i);
-        const V3 F = wi > 0.0f ? V3{0.0f + s.gx, 0.0f + s.gy, 0.0f + s.gz} : V3{0.0f, 0.0f, 0.0f};
-        v = {v.x * s.gd, v.y * s.gd, v.z * s.gd};
-        v = {v.x + (F.x * s.dt) * wi, v.y + (F.y * s.dt) * wi, v.z + (F.z * s.dt) * wi};
-        st3(c.v, c.N, i, v);
+        const V3 v = external_velocity(c, s, i);
         t[0] = x.x * s.mass;
         t[1] = x.y * s.mass;
         t[2] = x.z * s.mass;
@@ -115,13 +122,12 @@ __global__ __launch_bounds__(NB) void k_pbd_pass_b(DevPbd c, PbdScalars s) {
     block_partials<9>(t, c.part, c.nblk, 7);
 }
 
-__global__ __launch_bounds__(NB) void k_pbd_pass_c(DevPbd c, PbdScalars s) {
-    const int i = blockIdx.x * NB + threadIdx.x;
-    if (i >= c.N) return;
-    const float* S = c.sums;
-    // I = identity + sum (symmetric, column-major m[c][r]); glm::inverse
-    const float xx = 1.0f + S[10], yy = 1.0f + S[11], zz = 1.0f + S[12];
-    const float xy = 0.0f + S[13], xz = 0.0f + S[14], yz = 0.0f + S[15];
+// Centre-of-mass damping and prediction of particle i (PBD:633-668) from the
+// sums: I6 = I (xx yy zz xy xz yz), L = sum cross(R, m V), the centres xcm,
+// vcm.  I is inverted as glm::inverse does (every thread alike).
+__device__ inline void damp_predict(const DevPbd& c, const PbdScalars& s, int i, const float* I6,
+                                    const float* Lv, V3 xcm, V3 vcm) {
+    const float xx = I6[0], yy = I6[1], zz = I6[2], xy = I6[3], xz = I6[4], yz = I6[5];
     const float M[3][3] = {{xx, xy, xz}, {xy, yy, yz}, {xz, yz, zz}};
     const float det = +M[0][0] * (M[1][1] * M[2][2] - M[2][1] * M[1][2]) -
                       M[1][0] * (M[0][1] * M[2][2] - M[2][1] * M[0][2]) +
@@ -140,13 +146,12 @@ __global__ __launch_bounds__(NB) void k_pbd_pass_c(DevPbd c, PbdScalars s) {
     for (int a = 0; a < 3; a++)
 #pragma unroll
         for (int b = 0; b < 3; b++) Inv[a][b] = Inv[a][b] / det;
-    const float L[3] = {S[7], S[8], S[9]};
+    const float L[3] = {Lv[0], Lv[1], Lv[2]};
     V3 w;
     w.x = Inv[0][0] * L[0] + Inv[1][0] * L[1] + Inv[2][0] * L[2];
     w.y = Inv[0][1] * L[0] + Inv[1][1] * L[1] + Inv[2][1] * L[2];
     w.z = Inv[0][2] * L[0] + Inv[1][2] * L[1] + Inv[2][2] * L[2];
 
-    const V3 xcm = centre(S, 0), vcm = centre(S, 3);
     const V3 x = ld3(c.x, c.N, i);
     V3 v = ld3(c.v, c.N, i);
     const V3 R = {x.x - xcm.x, x.y - xcm.y, x.z - xcm.z};
@@ -158,10 +163,66 @@ __global__ __launch_bounds__(NB) void k_pbd_pass_c(DevPbd c, PbdScalars s) {
     st3(c.t, c.N, i, t);
 }
 
-__global__ __launch_bounds__(256) void k_pbd_distance(DevPbd c, int first, int count) {
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= count) return;
-    const int k = first + j;
+__global__ __launch_bounds__(NB) void k_pbd_pass_c(DevPbd c, PbdScalars s) {
+    const int i = blockIdx.x * NB + threadIdx.x;
+    if (i >= c.N) return;
+    const float* S = c.sums;
+    // I = identity + sum (symmetric, column-major m[c][r])
+    const float I6[6] = {1.0f + S[10], 1.0f + S[11], 1.0f + S[12], 0.0f + S[13], 0.0f + S[14], 0.0f + S[15]};
+    damp_predict(c, s, i, I6, S + 7, centre(S, 0), centre(S, 3));
+}
+
+// The prediction with the reference's sequential sums (list order): one
+// workgroup; thread 0 adds the centre-of-mass sums in particle order and
+// accumulates I onto the identity, as PBD:612-650 does; every particle's
+// update runs in parallel.
+constexpr int kSeqThreads = 1024;
+__global__ __launch_bounds__(kSeqThreads) void k_pbd_predict_seq(DevPbd c, PbdScalars s) {
+    __shared__ float sx[3], sv[3], sI[6], sL[3];
+    for (int i = threadIdx.x; i < c.N; i += kSeqThreads) external_velocity(c, s, i);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float xs[3] = {0.0f, 0.0f, 0.0f}, vs[3] = {0.0f, 0.0f, 0.0f}, ms = 0.0f;
+        for (int i = 0; i < c.N; i++) {
+            const V3 x = ld3(c.x, c.N, i), v = ld3(c.v, c.N, i);
+            xs[0] += x.x * s.mass;
+            xs[1] += x.y * s.mass;
+            xs[2] += x.z * s.mass;
+            vs[0] += v.x * s.mass;
+            vs[1] += v.y * s.mass;
+            vs[2] += v.z * s.mass;
+            ms += s.mass;
+        }
+        const V3 xcm = {xs[0] / ms, xs[1] / ms, xs[2] / ms};
+        float L[3] = {0.0f, 0.0f, 0.0f}, I6[6] = {1.0f, 1.0f, 1.0f, 0.0f, 0.0f, 0.0f};
+        for (int i = 0; i < c.N; i++) {
+            const V3 x = ld3(c.x, c.N, i), v = ld3(c.v, c.N, i);
+            const V3 R = {x.x - xcm.x, x.y - xcm.y, x.z - xcm.z};
+            const V3 l = cross3(R, V3{s.mass * v.x, s.mass * v.y, s.mass * v.z});
+            L[0] += l.x;
+            L[1] += l.y;
+            L[2] += l.z;
+            I6[0] += (R.y * R.y + R.z * R.z) * s.mass;
+            I6[1] += (R.x * R.x + R.z * R.z) * s.mass;
+            I6[2] += (R.x * R.x + R.y * R.y) * s.mass;
+            I6[3] += -(R.x * R.y) * s.mass;
+            I6[4] += -(R.x * R.z) * s.mass;
+            I6[5] += -(R.y * R.z) * s.mass;
+        }
+        for (int k = 0; k < 3; k++) {
+            sx[k] = k == 0 ? xcm.x : k == 1 ? xcm.y : xcm.z;
+            sv[k] = vs[k] / ms;
+            sL[k] = L[k];
+        }
+        for (int k = 0; k < 6; k++) sI[k] = I6[k];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < c.N; i += kSeqThreads)
+        damp_predict(c, s, i, sI, sL, V3{sx[0], sx[1], sx[2]}, V3{sv[0], sv[1], sv[2]});
+}
+
+// UpdateDistanceConstraint (PBD:680-701) of stored constraint k.
+__device__ inline void project_distance(const DevPbd& c, int k) {
     const int a = c.d_p1[k], b = c.d_p2[k];
     const float rest = c.d_rest[k], kp = c.d_kp[k];
     const float w1 = c.w[a], w2 = c.w[b];
@@ -177,10 +238,13 @@ __global__ __launch_bounds__(256) void k_pbd_distance(DevPbd c, int first, int c
     if (w2 > 0.0f) st3(c.t, c.N, b, V3{tb.x + dP.x * w2, tb.y + dP.y * w2, tb.z + dP.z * w2});
 }
 
-__global__ __launch_bounds__(256) void k_pbd_bending(DevPbd c, PbdScalars s, int first, int count) {
+__global__ __launch_bounds__(256) void k_pbd_distance(DevPbd c, int first, int count) {
     const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= count) return;
-    const int k = first + j;
+    if (j < count) project_distance(c, first + j);
+}
+
+// UpdateBendingConstraint (PBD:703-731) of stored constraint k.
+__device__ inline void project_bending(const DevPbd& c, const PbdScalars& s, int k) {
     const int a = c.b_p1[k], b = c.b_p2[k], d = c.b_p3[k];
     const float rest = c.b_rest[k], wsum = c.b_w[k], kp = c.b_kp[k];
     const float wa = c.w[a], wb = c.w[b], wd = c.w[d];
@@ -197,6 +261,41 @@ __global__ __launch_bounds__(256) void k_pbd_bending(DevPbd c, PbdScalars s, int
     if (wa > 0.0f) st3(c.t, c.N, a, V3{ta.x + sa * f.x, ta.y + sa * f.y, ta.z + sa * f.z});
     if (wb > 0.0f) st3(c.t, c.N, b, V3{tb.x + sb * f.x, tb.y + sb * f.y, tb.z + sb * f.z});
     if (wd > 0.0f) st3(c.t, c.N, d, V3{td.x + sd * f.x, td.y + sd * f.y, td.z + sd * f.z});
+}
+
+__global__ __launch_bounds__(256) void k_pbd_bending(DevPbd c, PbdScalars s, int first, int count) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j < count) project_bending(c, s, first + j);
+}
+
+// The list-order schedule in one workgroup.  The constraints of a level
+// touch disjoint particles and every earlier constraint sharing a particle
+// with one of them sits in an earlier level, so projecting the levels in
+// order, a level's constraints in parallel, gives the sequential list order's
+// results (UpdateInternalConstraints, PBD:876-889).  The workgroup barrier
+// after each level orders the global memory accesses of the one workgroup.
+constexpr int kLevelThreads = 1024;
+__global__ __launch_bounds__(kLevelThreads) void k_pbd_levels(DevPbd c, PbdScalars s, int iterations,
+                                                             int ground) {
+    for (int it = 0; it < iterations; it++) {
+        for (int L = 0; L < c.nd_lv; L++) {
+            const int first = c.d_lv[L], last = c.d_lv[L + 1];
+            for (int k = first + (int)threadIdx.x; k < last; k += kLevelThreads) project_distance(c, k);
+            __syncthreads();
+        }
+        for (int L = 0; L < c.nb_lv; L++) {
+            const int first = c.b_lv[L], last = c.b_lv[L + 1];
+            for (int k = first + (int)threadIdx.x; k < last; k += kLevelThreads) project_bending(c, s, k);
+            __syncthreads();
+        }
+        if (ground) {  // GroundCollision (PBD:839-845)
+            for (int i = threadIdx.x; i < c.N; i += kLevelThreads) {
+                float* ty = c.t + c.N + i;
+                if (*ty < 0.0f) *ty = 0.0f;
+            }
+            __syncthreads();
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_pbd_ground(DevPbd c) {
@@ -236,6 +335,17 @@ hipError_t launch_pbd_bending(const DevPbd& c, const PbdScalars& s, int first, i
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_pbd_bending, dim3((count + 255) / 256), dim3(256), 0, st, c, s, first,
                        count);
+    return hipGetLastError();
+}
+
+hipError_t launch_pbd_predict_seq(const DevPbd& c, const PbdScalars& s, hipStream_t st) {
+    hipLaunchKernelGGL(k_pbd_predict_seq, dim3(1), dim3(kSeqThreads), 0, st, c, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_pbd_levels(const DevPbd& c, const PbdScalars& s, int iterations, int ground,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(k_pbd_levels, dim3(1), dim3(kLevelThreads), 0, st, c, s, iterations, ground);
     return hipGetLastError();
 }
 

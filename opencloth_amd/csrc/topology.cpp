@@ -211,6 +211,45 @@ int build_grid_topology(int nx, int ny, const oc_spring* springs, int n_springs,
         }
     }
     if (!t.stencil) t.pair_mask.clear();
+
+    // stencil assembly form: each pair at most two springs; the owner (the
+    // pair's lower vertex) holds the constants of its pairs in owner planes
+    t.stencil_asm = t.stencil;
+    if (t.stencil_asm) {
+        t.asm_mask.assign(N, 0);
+        t.spring_cst.assign(S, -1);
+        std::vector<int> count(P, 0);
+        for (int s = 0; s < S && t.stencil_asm; s++)
+            if (++count[t.spring_pair[s]] > 2) t.stencil_asm = false;
+        std::vector<int> seen(P, 0);
+        for (int s = 0; s < S && t.stencil_asm; s++) {  // springs in id order
+            const int p = t.spring_pair[s];
+            const int a = t.pair_a[p], b = t.pair_b[p];
+            const int ddi = (b % nx) - (a % nx), ddj = (b / nx) - (a / nx);
+            int o = -1;
+            for (int q = 0; q < kOwned; q++)
+                if (stencil_di(owned_k(q)) == ddi && stencil_dj(owned_k(q)) == ddj) o = q;
+            if (o < 0) {
+                t.stencil_asm = false;
+                break;
+            }
+            const int m = seen[p]++;
+            t.spring_cst[s] = (int32_t)(((size_t)(m * kOwned + o) * 3) * N + a);
+            t.asm_mask[a] |= 1u << owned_k(o);
+            t.asm_mask[b] |= 1u << owned_partner_k(o);
+            if (m == 1) {
+                t.asm_mask[a] |= 1u << (16 + owned_k(o));
+                t.asm_mask[b] |= 1u << (16 + owned_partner_k(o));
+            }
+        }
+        // the masks' presence bits are the SpMV's stencil mask
+        for (int i = 0; i < N && t.stencil_asm; i++)
+            if ((t.asm_mask[i] & 0xFFFu) != t.pair_mask[i]) t.stencil_asm = false;
+    }
+    if (!t.stencil_asm) {
+        t.asm_mask.clear();
+        t.spring_cst.clear();
+    }
     return OC_OK;
 }
 

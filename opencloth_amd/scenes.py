@@ -154,6 +154,8 @@ class StepParams:
     ground_clamp: bool = False
     cg_persistent: bool = False  # all CG iterations in one cooperative launch
     cg_jacobi: bool = False      # Jacobi-preconditioned CG (Baraff-Witkin modified PCG)
+    damping: float = 0.0         # velocity damping c: force c*v on free vertices (BW:708)
+    cg_single_reduction: bool = False  # one reduction point per CG iteration (Chronopoulos-Gear)
 
 
 @dataclass
@@ -358,6 +360,27 @@ def demo_scene(n: int = 21) -> Scene:
                  extra={"ellipsoid": ellipsoid(), "provot_slack": 1.0})
 
 
+def bw_demo_scene(n: int = 21) -> Scene:
+    """The Baraff-Witkin demo's scene (OpenCloth_Implicit/.../main.cpp,
+    BW:60-128, BW:495-590): 21x21 at height fullsize+1 = 5 (BW:500), Ks
+    0.75/0.75/0.95, Kd -0.25 (BW:105-107), mass 1, gravity force -0.00981
+    (BW:108-109; an acceleration at mass 1), dt 5/60 (BW:65), velocity
+    damping DEFAULT_DAMPING -0.125 on every free vertex (BW:706-708), 10 CG
+    iterations to the relative tolerance 1e-3 (BW:126-128, BW:292) with the
+    Jacobi preconditioner BW:784-785 means, ground clamp (BW:790-792), pins
+    {0, numX}; the post-step passes use the same collider (BW:583-586) and
+    Provot slack 1.01 (BW:808).  BW's own step multiplies dv by dt twice
+    (BW:789) and leaves the damping out of its Jacobian; neither is copied."""
+    X = grid_positions(n, n, 4.0, 5.0)
+    springs = grid_springs(n, n, X, ks=(0.75, 0.75, 0.95), kd=(-0.25, -0.25, -0.25))
+    pinned = np.zeros(n * n, np.uint8)
+    pinned[0] = pinned[n - 1] = 1
+    p = StepParams(mass=1.0, gravity=(0.0, -0.00981, 0.0), dt=5.0 / 60.0, cg_max_iters=10, cg_tol=1e-3,
+                   cg_tol_mode=TOL_RELATIVE, ground_clamp=True, cg_jacobi=True, damping=-0.125)
+    return Scene(f"bw_demo_{n}x{n}", n, n, X, np.zeros_like(X), springs, pinned, p, 600,
+                 extra={"ellipsoid": ellipsoid(), "provot_slack": 1.01})
+
+
 def verlet_demo_scene(n: int = 21) -> Scene:
     """The Verlet demo's scene (VL:59-106, VL:240-328): height 5, Ks 50.75 /
     50.75 / 50.95, Kd -0.25, mass 1, gravity -0.00981, damping -0.0125,
@@ -440,6 +463,7 @@ class PBDParams:
     k_damp: float = 0.00125         # PBD:111
     solver_iterations: int = 2      # PBD:107
     ground: bool = True             # GroundCollision in the solver loop (PBD:887)
+    list_order: bool = False        # the reference's sequential order (PBD:876-889), one workgroup
 
 
 @dataclass

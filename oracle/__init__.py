@@ -20,6 +20,7 @@ REF = os.path.join(HERE, "_ref")
 
 NOT_CONVERGED, BREAKDOWN, NONFINITE = 1, 2, 4
 JACOBI = 4  # Params.flags bit: Jacobi-preconditioned CG
+CG1 = 16    # Params.flags bit: single-reduction CG (Chronopoulos-Gear)
 
 
 class Params(C.Structure):
@@ -28,7 +29,7 @@ class Params(C.Structure):
         ("h", C.c_double), ("tol", C.c_double),
         ("max_it", C.c_int32), ("tol_mode", C.c_int32), ("warm_start", C.c_int32),
         ("dot_mode", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("tx", C.c_int32),
-        ("ty", C.c_int32), ("flags", C.c_int32), ("pad", C.c_int32),
+        ("ty", C.c_int32), ("flags", C.c_int32), ("pad", C.c_int32), ("damping", C.c_double),
     ]
 
 
@@ -69,8 +70,8 @@ class Oracle:
         L.orc_grid_positions.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, _P]
         L.orc_grid_springs.argtypes = [C.c_int, C.c_int, _P] + [C.c_float] * 6 + [_P]
         L.orc_spring_terms.argtypes = [_P, _P, _P, _P, _P, _P]
-        L.orc_forces_scatter.argtypes = [_P, _P, _P, C.c_double, _P, _P]
-        L.orc_forces_gather.argtypes = [_P, _P, _P, C.c_double, _P, _P, _P]
+        L.orc_forces_scatter.argtypes = [_P, _P, _P, C.c_double, _P, C.c_double, _P]
+        L.orc_forces_gather.argtypes = [_P, _P, _P, C.c_double, _P, C.c_double, _P, _P]
         L.orc_assemble.argtypes = [_P, _P, _P, _P, _P, _P, _P]
         L.orc_matvec.argtypes = [_P, _P, C.c_double, _P, _P]
         L.orc_dot.restype = C.c_double
@@ -162,7 +163,9 @@ def make_params(p, nx=0, ny=0, tx=0, ty=0, dot_mode=0, **over) -> Params:
     q.dot_mode = dot_mode
     q.nx, q.ny, q.tx, q.ty = nx, ny, tx, ty
     q.flags = (1 if over.get("ground_clamp", p.ground_clamp) else 0) | (
-        JACOBI if over.get("cg_jacobi", getattr(p, "cg_jacobi", False)) else 0)
+        JACOBI if over.get("cg_jacobi", getattr(p, "cg_jacobi", False)) else 0) | (
+        CG1 if over.get("cg_single_reduction", getattr(p, "cg_single_reduction", False)) else 0)
+    q.damping = float(over.get("damping", getattr(p, "damping", 0.0)))
     return q
 
 
@@ -199,20 +202,21 @@ class System:
         o.L.orc_spring_terms(self.h, _ptr(X), _ptr(V), _ptr(F), _ptr(K), _ptr(P))
         return F, K, P
 
-    def forces_scatter(self, X, V, mass, g):
+    def forces_scatter(self, X, V, mass, g, damping=0.0):
         o = self.o
         X = o.arr(X); V = o.arr(V)
         F = np.zeros((self.N, 3), o.dtype)
         gg = np.asarray(g, np.float64)
-        o.L.orc_forces_scatter(self.h, _ptr(X), _ptr(V), float(mass), _ptr(gg), _ptr(F))
+        o.L.orc_forces_scatter(self.h, _ptr(X), _ptr(V), float(mass), _ptr(gg), float(damping), _ptr(F))
         return F
 
-    def forces_gather(self, X, V, mass, g):
+    def forces_gather(self, X, V, mass, g, damping=0.0):
         o = self.o
         X = o.arr(X); V = o.arr(V)
         F = np.zeros((self.N, 3), o.dtype); KV = np.zeros((self.N, 3), o.dtype)
         gg = np.asarray(g, np.float64)
-        o.L.orc_forces_gather(self.h, _ptr(X), _ptr(V), float(mass), _ptr(gg), _ptr(F), _ptr(KV))
+        o.L.orc_forces_gather(self.h, _ptr(X), _ptr(V), float(mass), _ptr(gg), float(damping), _ptr(F),
+                              _ptr(KV))
         return F, KV
 
     def assemble(self, X, V, prm: Params, with_diag=False):

@@ -93,7 +93,15 @@ typedef struct {
     int32_t nx, ny, tx, ty; /* grid and tile shape (dot_mode 1) */
     int32_t flags;      /* bit0: clamp y<0 to 0 after the update (IE:695-697) */
     int32_t pad;
+    double damping;     /* velocity damping c: f_i += c*v_i on free vertices (BW:706-708
+                           DEFAULT_DAMPING, negative damps), with its Jacobian c*I in A */
 } orc_params;
+
+/* Diagonal mass of A: m - h*c, the velocity damping's term of
+ * A = M - h df/dv - h^2 df/dx (m when c = 0, exactly). */
+static real diag_mass(const orc_params* prm) {
+    return (real)prm->mass - (real)prm->h * (real)prm->damping;
+}
 
 #define ORC_NOT_CONVERGED 1
 #define ORC_BREAKDOWN 2
@@ -102,6 +110,7 @@ typedef struct {
 /* orc_params.flags */
 #define ORC_FLAG_GROUND_CLAMP 1
 #define ORC_FLAG_JACOBI 4 /* Jacobi-preconditioned CG (Baraff-Witkin modified PCG) */
+#define ORC_FLAG_CG1 16   /* single-reduction CG (Chronopoulos and Gear 1989) */
 
 /* ------------------------------------------------------------------------ */
 /* vec3 / symmetric-block helpers in GLM evaluation order                    */
@@ -435,13 +444,16 @@ EXPORT void orc_spring_terms(const orc_sys* s, const real* X, const real* V, rea
  * Pinned by: equality with orc_forces_gather (an independent loop order) and
  * the physics tests of spring_terms. */
 EXPORT void orc_forces_scatter(const orc_sys* s, const real* X, const real* V, double mass,
-                               const double* g, real* F) {
-    real m = (real)mass;
+                               const double* g, double damping, real* F) {
+    real m = (real)mass, dmp = (real)damping;
     real gv[3] = {(real)g[0], (real)g[1], (real)g[2]};
     for (int i = 0; i < s->N; i++) {
         F[3 * i + 0] = F[3 * i + 1] = F[3 * i + 2] = 0;
         if (!s->pinned[i])
             for (int c = 0; c < 3; c++) F[3 * i + c] += m * gv[c];
+        /* BW:708 adds DEFAULT_DAMPING*V[i] after gravity */
+        if (!s->pinned[i] && damping != 0)
+            for (int c = 0; c < 3; c++) F[3 * i + c] += dmp * V[3 * i + c];
     }
     for (int i = 0; i < s->S; i++) {
         const orc_spring* sp = &s->springs[i];
@@ -459,13 +471,15 @@ EXPORT void orc_forces_scatter(const orc_sys* s, const real* X, const real* V, d
  * pair-grouped order (the order the HIP assembly kernel uses).  Also returns
  * KV = (df/dx) v, the stiffness Jacobian applied to the velocities. */
 EXPORT void orc_forces_gather(const orc_sys* s, const real* X, const real* V, double mass,
-                              const double* g, real* F, real* KV) {
-    real m = (real)mass;
+                              const double* g, double damping, real* F, real* KV) {
+    real m = (real)mass, dmp = (real)damping;
     real gv[3] = {(real)g[0], (real)g[1], (real)g[2]};
     for (int i = 0; i < s->N; i++) {
         real acc[3] = {0, 0, 0}, kv[3] = {0, 0, 0};
         if (!s->pinned[i])
             for (int c = 0; c < 3; c++) acc[c] += m * gv[c];
+        if (!s->pinned[i] && damping != 0)
+            for (int c = 0; c < 3; c++) acc[c] += dmp * V[3 * i + c];
         for (int e = s->inc_ptr[i]; e < s->inc_ptr[i + 1]; e++) {
             const orc_spring* sp = &s->springs[s->inc_s[e]];
             real f[3], K[6];
@@ -497,8 +511,9 @@ EXPORT void orc_forces_gather(const orc_sys* s, const real* X, const real* V, do
 /* ------------------------------------------------------------------------ */
 /* Off-diagonal block of pair (i,j):  C_ij = sum_s (h*Kd_s*P_s + h^2*K_s), the
  * sum over the pair's springs in list order.  Diagonal block:
- * A_ii = m*I - sum_j C_ij, never stored: A d is evaluated as
- *   (A d)_i = m*d_i + sum_j C_ij (d_j - d_i)      (pairs in vertex order).
+ * A_ii = (m - h*c)*I - sum_j C_ij, never stored (c the velocity damping,
+ * usually 0; diag_mass): A d is evaluated as
+ *   (A d)_i = (m - h*c)*d_i + sum_j C_ij (d_j - d_i)      (pairs in vertex order).
  * df/dv of one spring is Kd*P (the derivative of the force the reference
  * applies, IE:674); BW:768-770's -Kd*(dp1+dp2+dp3) has the opposite sign
  * with BW's own negative Kd and is not followed.  df/dx is the reference's
@@ -520,7 +535,8 @@ static void spring_block(const orc_sys* s, const orc_spring* sp, const real* X, 
  * B against an explicit dense evaluation. */
 EXPORT void orc_assemble(const orc_sys* s, const real* X, const real* V, const orc_params* prm,
                          real* B, real* Cp, real* diag) {
-    real h = (real)prm->h, hh = h * h, m = (real)prm->mass;
+    real h = (real)prm->h, hh = h * h, m = (real)prm->mass, md = diag_mass(prm);
+    real dmp = (real)prm->damping;
     real gv[3] = {(real)prm->gx, (real)prm->gy, (real)prm->gz};
     /* pair blocks, in list order within each pair (each spring's block is
      * evaluated first, then the blocks are added serially) */
@@ -542,6 +558,8 @@ EXPORT void orc_assemble(const orc_sys* s, const real* X, const real* V, const o
         real acc[3] = {0, 0, 0}, kv[3] = {0, 0, 0};
         if (!s->pinned[i])
             for (int c = 0; c < 3; c++) acc[c] += m * gv[c];
+        if (!s->pinned[i] && prm->damping != 0)
+            for (int c = 0; c < 3; c++) acc[c] += dmp * V[3 * i + c];
         for (int e = s->inc_ptr[i]; e < s->inc_ptr[i + 1]; e++) {
             const orc_spring* sp = &s->springs[s->inc_s[e]];
             real f[3], K[6], C[6];
@@ -564,7 +582,7 @@ EXPORT void orc_assemble(const orc_sys* s, const real* X, const real* V, const o
             B[3 * i + c] = s->pinned[i] ? (real)0 : h * t;
         }
         if (diag) {
-            real dg[3] = {m, m, m};
+            real dg[3] = {md, md, md};
             for (int k = s->vp_ptr[i]; k < s->vp_ptr[i + 1]; k++) {
                 const real* C = Cp + 6 * s->vp_pair[k];
                 for (int c = 0; c < 3; c++) dg[c] -= C[c];
@@ -574,7 +592,8 @@ EXPORT void orc_assemble(const orc_sys* s, const real* X, const real* V, const o
     }
 }
 
-/* q = S A d with the on-the-fly diagonal (see above); S zeroes pinned rows.
+/* q = S A d with the on-the-fly diagonal (see above; `mass` is the diagonal
+ * mass m - h*c); S zeroes pinned rows.
  * Pinned by: the dense-matrix tests (q equals the dense A times d). */
 EXPORT void orc_matvec(const orc_sys* s, const real* Cp, double mass, const real* d, real* q) {
     real m = (real)mass;
@@ -749,6 +768,102 @@ static int cg_core(int N, const uint8_t* pinned, matvec_fn mv, dot_fn dot, const
     return it;
 }
 
+/* The same solve with one reduction point per iteration: the CG variant of
+ * A. T. Chronopoulos and C. W. Gear, "s-step iterative methods for
+ * symmetric linear systems" (J. Comput. Appl. Math. 25, 1989), s = 1.  With
+ * u = P^-1 r and w = S A u, both dots gamma = r.u and delta = w.u come from
+ * the same vectors, and
+ *   beta_k  = gamma_k / gamma_{k-1}                      (0 at k = 0)
+ *   alpha_k = gamma_k / (delta_k - beta_k gamma_k / alpha_{k-1})
+ *   p_k = u_k + beta_k p_{k-1},  s_k = w_k + beta_k s_{k-1}   (s = S A p)
+ *   dv += alpha_k p_k,  r_{k+1} = r_k - alpha_k s_k,  u_{k+1} = P^-1 r_{k+1},
+ *   w_{k+1} = S A u_{k+1}.
+ * In exact arithmetic it is the recurrence of cg_core (the denominator is
+ * p_k.A p_k, by the A-conjugacy of p and the orthogonality of the
+ * residuals); in floating point the iterates differ by rounding.  The stop
+ * tests use gamma (= delta of cg_core).  Guards as in cg_core: a denominator
+ * < 0 or non-finite stops with ORC_BREAKDOWN, a zero one gives alpha = 0,
+ * gamma_{k-1} == 0 gives beta = 0, and alpha_{k-1} == 0 drops the
+ * correction term (both only after the residual underflowed).
+ * Pinned by: the FP64 dense solve; agreement with cg_core to 1e-4 (FP32) and
+ * 1e-10 (FP64) on converged solves; a Python transcription of the
+ * published recurrence on the dense filtered matrix, every iterate. */
+static int cg_core_cg1(int N, const uint8_t* pinned, matvec_fn mv, dot_fn dot, const void* ctx,
+                       int max_it, real tol, int tol_mode, const real* pinv, const real* b, real* dv,
+                       int* status, double* out_delta) {
+    const size_t n3 = 3 * (size_t)(N > 0 ? N : 1);
+    real* r = (real*)malloc(sizeof(real) * n3);
+    real* w = (real*)malloc(sizeof(real) * n3);
+    real* p = (real*)malloc(sizeof(real) * n3);
+    real* sv = (real*)malloc(sizeof(real) * n3);
+    real* u = pinv ? (real*)malloc(sizeof(real) * n3) : r; /* P^-1 r */
+    mv(ctx, dv, w);
+    OMP_FOR
+    for (int i = 0; i < 3 * N; i++) {
+        r[i] = (pinned && pinned[i / 3]) ? (real)0 : b[i] - w[i];
+        if (pinv) u[i] = pinv[i] * r[i];
+    }
+    mv(ctx, u, w);
+    real tol2 = tol * tol;
+    real gamma = dot(ctx, r, u), delta = dot(ctx, w, u);
+    real gamma0 = gamma;
+    real thresh = tol_mode == 1 ? tol2 * gamma0 : tol2;
+    real gamma_old = 0, alpha_old = 0;
+    int it = 0, st = 0;
+    while (it < max_it) {
+        if (tol_mode != 2 && !(gamma > thresh)) break;
+        real beta = (it > 0 && gamma_old != 0) ? gamma / gamma_old : (real)0;
+        real den = delta;
+        if (beta != 0 && alpha_old != 0) {
+            real t = beta * gamma;
+            t = t / alpha_old;
+            den = delta - t;
+        }
+        if (den < 0 || !ISFINITE_(den)) {
+            st |= ORC_BREAKDOWN;
+            break;
+        }
+        real alpha = den == 0 ? (real)0 : gamma / den;
+        const int first = it == 0;
+        OMP_FOR
+        for (int i = 0; i < 3 * N; i++) {
+            real ui = pinv ? pinv[i] * r[i] : r[i];
+            p[i] = first ? ui : ui + beta * p[i];
+            sv[i] = first ? w[i] : w[i] + beta * sv[i];
+            dv[i] = dv[i] + alpha * p[i];
+            r[i] = r[i] - alpha * sv[i];
+            if (pinv) u[i] = pinv[i] * r[i];
+        }
+        mv(ctx, u, w);
+        gamma_old = gamma;
+        alpha_old = alpha;
+        gamma = dot(ctx, r, u);
+        delta = dot(ctx, w, u);
+        it++;
+    }
+    if (tol_mode != 2 && gamma > thresh && !(st & ORC_BREAKDOWN)) st |= ORC_NOT_CONVERGED;
+    if (!ISFINITE_(gamma)) st |= ORC_NONFINITE;
+    if (status) *status = st;
+    if (out_delta) *out_delta = (double)gamma;
+    if (pinv) free(u);
+    free(r);
+    free(w);
+    free(p);
+    free(sv);
+    return it;
+}
+
+/* cg_core or, with ORC_FLAG_CG1, cg_core_cg1. */
+static int cg_solve(const orc_params* prm, int N, const uint8_t* pinned, matvec_fn mv, dot_fn dot,
+                    const void* ctx, const real* pinv, const real* b, real* dv, int* status,
+                    double* out_delta) {
+    if (prm->flags & ORC_FLAG_CG1)
+        return cg_core_cg1(N, pinned, mv, dot, ctx, prm->max_it, (real)prm->tol, prm->tol_mode, pinv, b,
+                           dv, status, out_delta);
+    return cg_core(N, pinned, mv, dot, ctx, prm->max_it, (real)prm->tol, prm->tol_mode, pinv, b, dv,
+                   status, out_delta);
+}
+
 typedef struct {
     const orc_sys* s;
     const real* Cp;
@@ -757,7 +872,7 @@ typedef struct {
 
 static void pair_mv(const void* ctx, const real* in, real* out) {
     const pair_ctx* c = (const pair_ctx*)ctx;
-    orc_matvec(c->s, c->Cp, c->prm->mass, in, out);
+    orc_matvec(c->s, c->Cp, (double)diag_mass(c->prm), in, out);
 }
 static real pair_dot(const void* ctx, const real* a, const real* b) {
     const pair_ctx* c = (const pair_ctx*)ctx;
@@ -772,7 +887,7 @@ static real pair_dot(const void* ctx, const real* a, const real* b) {
 static real* jacobi_inverse(const orc_sys* s, const real* Cp, const orc_params* prm) {
     if (!(prm->flags & ORC_FLAG_JACOBI)) return NULL;
     real* pinv = (real*)malloc(sizeof(real) * 3 * (s->N > 0 ? s->N : 1));
-    real m = (real)prm->mass;
+    real m = diag_mass(prm);
     OMP_FOR
     for (int i = 0; i < s->N; i++) {
         real dg[3] = {m, m, m};
@@ -788,8 +903,7 @@ EXPORT int orc_cg(const orc_sys* s, const real* Cp, const orc_params* prm, const
                   int* status, double* out_delta) {
     pair_ctx c = {s, Cp, prm};
     real* pinv = jacobi_inverse(s, Cp, prm);
-    int it = cg_core(s->N, s->pinned, pair_mv, pair_dot, &c, prm->max_it, (real)prm->tol,
-                     prm->tol_mode, pinv, b, dv, status, out_delta);
+    int it = cg_solve(prm, s->N, s->pinned, pair_mv, pair_dot, &c, pinv, b, dv, status, out_delta);
     free(pinv);
     return it;
 }
@@ -899,7 +1013,7 @@ EXPORT int orc_ie_literal_step(const orc_sys* s, real* X, real* V, real* Vnew, d
     real* F = (real*)malloc(sizeof(real) * 3 * (N > 0 ? N : 1));
     real* A = (real*)malloc(sizeof(real) * 9 * (N > 0 ? N : 1));
     real* b = (real*)malloc(sizeof(real) * 3 * (N > 0 ? N : 1));
-    orc_forces_scatter(s, X, V, mass, g, F);
+    orc_forces_scatter(s, X, V, mass, g, 0.0, F);
     for (int i = 0; i < N; i++) {
         const orc_spring* sp = &s->springs[i];
         real f[3], K[6];
@@ -1057,12 +1171,15 @@ EXPORT int orc_step_adjoint(const orc_sys* s, const real* X, const real* V, cons
     orc_assemble(s, X, V, prm, B, Cp, NULL);
     pair_ctx pc = {s, Cp, prm};
     real* pinv = jacobi_inverse(s, Cp, prm);
-    int it = cg_core(N, s->pinned, pair_mv, pair_dot, &pc, prm->max_it, (real)prm->tol,
-                     prm->tol_mode, pinv, gvt, mu, status, NULL);
+    int it = cg_solve(prm, N, s->pinned, pair_mv, pair_dot, &pc, pinv, gvt, mu, status, NULL);
     free(pinv);
+    real dmp = (real)prm->damping;
     for (int i = 0; i < N; i++) {
         real gx[3] = {gx_out[3 * i], gx_out[3 * i + 1], gx_out[3 * i + 2]};
         real gv[3] = {gvt[3 * i], gvt[3 * i + 1], gvt[3 * i + 2]};
+        /* h mu.f with f_i += c v_i on free vertices: dJ/dv_i += h c mu_i */
+        if (!s->pinned[i] && prm->damping != 0)
+            for (int c = 0; c < 3; c++) gv[c] = gv[c] + h * (dmp * mu[3 * i + c]);
         for (int e = s->inc_ptr[i]; e < s->inc_ptr[i + 1]; e++) {
             int sid = s->inc_s[e];
             const orc_spring* sp = &s->springs[sid];

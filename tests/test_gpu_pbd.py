@@ -108,3 +108,90 @@ def test_partial_stiffness_and_more_iterations(o32):
     X, V = s.step(sc.X, sc.V, make_pbd_params(sc.params), 1, 30)
     Xg, Vg = cl.get_state()
     assert np.array_equal(bits(Xg), bits(X)) and np.array_equal(bits(Vg), bits(V))
+
+
+def levels_of(n_particles, constraints):
+    """Level of each constraint in list order: one more than the highest level
+    among the earlier constraints sharing a particle (a sequential scan)."""
+    last = np.full(n_particles, -1)
+    lev = np.zeros(len(constraints), np.int64)
+    for c, ps in enumerate(constraints):
+        lev[c] = max(last[p] for p in ps) + 1
+        for p in ps:
+            last[p] = lev[c]
+    return lev
+
+
+def test_list_order_levels_match(o32):
+    """OC_FLAG_LIST_ORDER: the level counts the library reports equal a
+    sequential scan of the lists; every level's constraints share no
+    particle and each constraint comes after every earlier one it touches."""
+    import dataclasses
+
+    from opencloth_amd.cloth import PBDCloth
+
+    sc = scenes.pbd_demo_scene()
+    sc.params = dataclasses.replace(sc.params, list_order=True)
+    cl = PBDCloth.from_scene(sc)
+    dl = levels_of(sc.n_vertices, [(c["p1"], c["p2"]) for c in sc.distance])
+    bl = levels_of(sc.n_vertices, [(c["p1"], c["p2"], c["p3"]) for c in sc.bending])
+    assert cl.num_colors() == (int(dl.max()) + 1, int(bl.max()) + 1)
+    for L in range(int(dl.max()) + 1):
+        ps = np.concatenate([[c["p1"], c["p2"]] for c in sc.distance[dl == L]])
+        assert len(np.unique(ps)) == len(ps)
+    cl.close()
+
+
+@pytest.mark.parametrize("steps", [1, 600])
+def test_list_order_equals_sequential_oracle(o32, steps):
+    """The reference's sequential order (list order and sequential sums, the
+    oracle's order 0) on the PBD demo with its collider: bit-exact after one
+    step and over 600 steps (checked every 100)."""
+    import dataclasses
+
+    from oracle import make_pbd_params
+    from opencloth_amd.cloth import PBDCloth
+
+    sc = scenes.pbd_demo_scene()
+    sc.params = dataclasses.replace(sc.params, list_order=True)
+    ell = sc.extra["ellipsoid"]
+    say("pbd list order x", steps)
+    cl = PBDCloth.from_scene(sc)
+    s = oracle_system(o32, sc)
+    prm = make_pbd_params(sc.params)
+    X, V = sc.X.copy(), sc.V.copy()
+    done = 0
+    while done < steps:
+        n = min(100, steps - done)
+        cl.step(n, ell)
+        X, V = s.step(X, V, prm, 0, n, ell)
+        done += n
+        Xg, Vg = cl.get_state()
+        assert np.array_equal(bits(Xg), bits(X)), done
+        assert np.array_equal(bits(Vg), bits(V)), done
+    if steps == 600:
+        assert X[:, 1].min() < 2.0  # draped on the collider
+    cl.close()
+
+
+def test_list_order_perturbed_64x64(o32):
+    """A perturbed 64x64 cloth with random velocities, 5 solver iterations:
+    10 steps in list order bit-exact against the oracle's order 0."""
+    import dataclasses
+
+    from oracle import make_pbd_params
+    from opencloth_amd.cloth import PBDCloth
+
+    sc = scenes.pbd_large_scene(64, seed=7)
+    sc.params = dataclasses.replace(sc.params, list_order=True, solver_iterations=5)
+    rng = np.random.default_rng(8)
+    V = (0.05 * rng.standard_normal(sc.X.shape)).astype(np.float32)
+    V[sc.pinned.astype(bool)] = 0
+    say("pbd list order 64x64 x10")
+    cl = PBDCloth(sc.X, sc.distance, sc.bending, sc.pinned, sc.params, V)
+    s = oracle_system(o32, sc)
+    cl.step(10)
+    X, Vr = s.step(sc.X, V, make_pbd_params(sc.params), 0, 10)
+    Xg, Vg = cl.get_state()
+    assert np.array_equal(bits(Xg), bits(X)) and np.array_equal(bits(Vg), bits(Vr))
+    cl.close()
