@@ -341,7 +341,9 @@ __device__ inline SpringOut eval_spring(V3 x1, V3 x2, V3 v1, V3 v2, float L0, fl
 template <int TX, int TY>
 struct AssembleStShared {
     static constexpr int WX = TX + 2 * H, WY = TY + 2 * H;
-    static constexpr int NR = TX * TY + strip_max<TX, TY>();
+    // columns 0 .. TX TY - 1: this direction's pairs owned in the tile;
+    // then the strips of every direction, all evaluated before the first
+    static constexpr int NR = TX * TY + strip_total<TX, TY>();
     float wx[3][WY][WX];
     float wv[3][WY][WX];
     float wdv[3][WY][WX];
@@ -358,7 +360,7 @@ __global__ __launch_bounds__(TX* TY, 4) void k_assemble_st(DevCloth c, StepScala
     using SH = AssembleStShared<TX, TY>;
     constexpr int WX = SH::WX, WY = SH::WY;
     constexpr int NWIN = (WX * WY + NT - 1) / NT;
-    static_assert(strip_max<TX, TY>() <= NT, "a strip must fit one round of the block");
+    static_assert(strip_total<TX, TY>() <= NT, "the strips must fit one round of the block");
     __shared__ SH sh;
     const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
     const int tile = tile_of_block(blockIdx.x, c.nt);
@@ -372,6 +374,28 @@ __global__ __launch_bounds__(TX* TY, 4) void k_assemble_st(DevCloth c, StepScala
             c.ctl[0] = 0;
             c.ctl[1] = 0;
             c.ctl[2] = 0;
+        }
+    }
+    auto own_cst = [&](int m, int o, int q, int g) {
+        return ldo(c.cst, (uint32_t)(((m * kOwned + o) * 3 + q) * N + g));
+    };
+    // this thread's strip column (owner outside the tile, partner inside):
+    // the strips of all directions are evaluated in one pass, packed into the
+    // first waves, so no phase below waits on a wave with two evaluations
+    int so = 0, swx = 0, swy = 0;
+    bool sin = false;
+    float sL0 = 0.0f, sks = 0.0f, skd = 0.0f;
+    if ((int)threadIdx.x < strip_total<TX, TY>()) {
+        int ss;
+        strip_of_column<TX, TY>(threadIdx.x, so, ss);
+        strip_owner<TX, TY>(so, ss, swx, swy);
+        const int gx = bx * TX + swx, gy = by * TY + swy;
+        if (gx >= 0 && gx < nx && gy >= 0 && gy < c.ny) {
+            const int g = gy * nx + gx;
+            sin = true;
+            sL0 = own_cst(0, so, 0, g);
+            sks = own_cst(0, so, 1, g);
+            skd = own_cst(0, so, 2, g);
         }
     }
     // the window: x, v, dv and the stencil masks of the tile and its halo
@@ -411,30 +435,13 @@ __global__ __launch_bounds__(TX* TY, 4) void k_assemble_st(DevCloth c, StepScala
             sh.wm[wy][wx] = mw;
         }
     }
-    // the first direction's constants: this vertex's pair and its strip owner's
-    auto own_cst = [&](int m, int o, int q, int g) {
-        return ldo(c.cst, (uint32_t)(((m * kOwned + o) * 3 + q) * N + g));
-    };
-    float oL0 = 0.0f, oks = 0.0f, okd = 0.0f, sL0 = 0.0f, sks = 0.0f, skd = 0.0f;
-    int swx = 0, swy = 0;
-    bool sin = false;  // this thread has a strip owner on the grid
+    // the first direction's constants of this vertex's pair
+    float oL0 = 0.0f, oks = 0.0f, okd = 0.0f;
     auto prefetch = [&](int o) {
         if (own) {
             oL0 = own_cst(0, o, 0, i);
             oks = own_cst(0, o, 1, i);
             okd = own_cst(0, o, 2, i);
-        }
-        sin = false;
-        if ((int)threadIdx.x < strip_count<TX, TY>(o)) {
-            strip_owner<TX, TY>(o, threadIdx.x, swx, swy);
-            const int gx = bx * TX + swx, gy = by * TY + swy;
-            if (gx >= 0 && gx < nx && gy >= 0 && gy < c.ny) {
-                const int g = gy * nx + gx;
-                sin = true;
-                sL0 = own_cst(0, o, 0, g);
-                sks = own_cst(0, o, 1, g);
-                skd = own_cst(0, o, 2, g);
-            }
         }
     };
     prefetch(0);
@@ -509,31 +516,32 @@ __global__ __launch_bounds__(TX* TY, 4) void k_assemble_st(DevCloth c, StepScala
         for (int q = 0; q < 6; q++) e.c[q] = sh.res[6 + q][r];
         return e;
     };
+    // ---- the strips: column NT + t holds strip column t's pair (the owner's side)
+    if (sin && ((sh.wm[swy + H][swx + H] >> owned_k(so)) & 1u)) {
+        const int px = swx + H, py = swy + H, dx = own_dx(so), dy = own_dy(so);
+        const SpringOut e = eval_spring(window3(sh.wx, px, py), window3(sh.wx, px + dx, py + dy),
+                                        window3(sh.wv, px, py), window3(sh.wv, px + dx, py + dy),
+                                        sL0, sks, skd, h, hh);
+        const int r = NT + threadIdx.x;
+        sh.res[0][r] = e.f.x;
+        sh.res[1][r] = e.f.y;
+        sh.res[2][r] = e.f.z;
+        sh.res[3][r] = e.kw.x;
+        sh.res[4][r] = e.kw.y;
+        sh.res[5][r] = e.kw.z;
+#pragma unroll
+        for (int q = 0; q < 6; q++) sh.res[6 + q][r] = e.c[q];
+    }
     SpringOut held{};  // this vertex's k = 7 pair, added after k = 5 and k = 6
 #pragma unroll 1
     for (int o = 0; o < kOwned; o++) {
         const int dx = own_dx(o), dy = own_dy(o);
         const int ko = owned_k(o), kp = owned_partner_k(o);
-        // ---- evaluate: this vertex's pair in direction o, then its strip owner's
+        // ---- evaluate this vertex's pair in direction o
         if (own && ((mi >> ko) & 1u)) {
             const SpringOut e = eval_spring(xi, window3(sh.wx, cx + dx, cy + dy), vi,
                                             window3(sh.wv, cx + dx, cy + dy), oL0, oks, okd, h, hh);
             const int r = threadIdx.x;
-            sh.res[0][r] = e.f.x;
-            sh.res[1][r] = e.f.y;
-            sh.res[2][r] = e.f.z;
-            sh.res[3][r] = e.kw.x;
-            sh.res[4][r] = e.kw.y;
-            sh.res[5][r] = e.kw.z;
-#pragma unroll
-            for (int q = 0; q < 6; q++) sh.res[6 + q][r] = e.c[q];
-        }
-        if (sin && ((sh.wm[swy + H][swx + H] >> ko) & 1u)) {
-            const int px = swx + H, py = swy + H;
-            const SpringOut e = eval_spring(window3(sh.wx, px, py), window3(sh.wx, px + dx, py + dy),
-                                            window3(sh.wv, px, py), window3(sh.wv, px + dx, py + dy),
-                                            sL0, sks, skd, h, hh);
-            const int r = NT + threadIdx.x;
             sh.res[0][r] = e.f.x;
             sh.res[1][r] = e.f.y;
             sh.res[2][r] = e.f.z;
@@ -548,8 +556,9 @@ __global__ __launch_bounds__(TX* TY, 4) void k_assemble_st(DevCloth c, StepScala
         // ---- accumulate, in this vertex's stencil order
         if (own && ((mi >> kp) & 1u)) {  // partner of the pair owned at -(dx, dy)
             const int wx = lx - dx, wy = ly - dy;
-            const int r = (wx >= 0 && wx < TX && wy >= 0 && wy < TY) ? wy * TX + wx
-                                                                       : NT + strip_index<TX, TY>(o, wx, wy);
+            const int r = (wx >= 0 && wx < TX && wy >= 0 && wy < TY)
+                              ? wy * TX + wx
+                              : NT + strip_base<TX, TY>(o) + strip_index<TX, TY>(o, wx, wy);
             SpringOut e = stored(r);
             e.f = {0.0f - e.f.x, 0.0f - e.f.y, 0.0f - e.f.z};
             e.kw = {0.0f - e.kw.x, 0.0f - e.kw.y, 0.0f - e.kw.z};

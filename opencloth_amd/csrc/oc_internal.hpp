@@ -73,9 +73,28 @@ __host__ __device__ constexpr int own_dy(int o) { return stencil_dj(owned_k(o));
 // rows above it; diagonals: the row above (TX owners) and the column on the
 // side the diagonal comes from (TY - 1 owners).
 template <int TX, int TY>
-__host__ __device__ inline int strip_count(int o) {
-    const int dx = own_dx(o), dy = own_dy(o);
-    return dy == 0 ? dx * TY : dx == 0 ? dy * TX : TX + TY - 1;
+__host__ __device__ constexpr int strip_count(int o) {
+    return own_dy(o) == 0 ? own_dx(o) * TY : own_dx(o) == 0 ? own_dy(o) * TX : TX + TY - 1;
+}
+// The strips of all owned directions one after another: direction o's strip
+// starts at column strip_base(o); strip_total() columns in all.
+template <int TX, int TY>
+__host__ __device__ constexpr int strip_base(int o) {
+    return (o > 0 ? strip_count<TX, TY>(0) : 0) + (o > 1 ? strip_count<TX, TY>(1) : 0) +
+           (o > 2 ? strip_count<TX, TY>(2) : 0) + (o > 3 ? strip_count<TX, TY>(3) : 0) +
+           (o > 4 ? strip_count<TX, TY>(4) : 0) + (o > 5 ? strip_count<TX, TY>(5) : 0);
+}
+template <int TX, int TY>
+__host__ __device__ constexpr int strip_total() {
+    static_assert(kOwned == 6, "strip_base sums six directions");
+    return strip_base<TX, TY>(kOwned);
+}
+// Direction o and strip index s of column t (0 <= t < strip_total()).
+template <int TX, int TY>
+__host__ __device__ inline void strip_of_column(int t, int& o, int& s) {
+    o = 0;
+    while (o + 1 < kOwned && t >= strip_base<TX, TY>(o + 1)) o++;
+    s = t - strip_base<TX, TY>(o);
 }
 template <int TX, int TY>
 __host__ __device__ constexpr int strip_max() {

@@ -1,16 +1,34 @@
 // Host check of the spring-once assembly's strip tables (oc_internal.hpp):
 // for every owned direction and every vertex of a tile, the LDS column a
 // partner reads is the one its pair's owner wrote, and every strip owner
-// lies in the tile's 2-wide halo with its partner inside the tile.  Prints
+// lies in the tile's 2-wide halo with its partner inside the tile; the
+// strips of all directions, laid out one after another, map back to their
+// direction and index, and for the kernel's tile fit one round of the block.  Prints
 // "ok" or the first mismatch.
 #include <cstdio>
 
 #include "oc_internal.hpp"
 
 template <int TX, int TY>
-int check() {
+int check(bool kernel_tile) {
     constexpr int H = oc::kHalo;
     if (oc::strip_max<TX, TY>() > TX * TY) return std::printf("strip larger than a block\n"), 1;
+    if (kernel_tile && oc::strip_total<TX, TY>() > TX * TY)
+        return std::printf("strips larger than a block\n"), 1;
+    // the strips of all directions laid out one after another
+    int total = 0;
+    for (int o = 0; o < oc::kOwned; o++) {
+        if (oc::strip_base<TX, TY>(o) != total) return std::printf("o %d: base\n", o), 1;
+        total += oc::strip_count<TX, TY>(o);
+    }
+    if (total != oc::strip_total<TX, TY>()) return std::printf("total %d\n", total), 1;
+    for (int t = 0; t < total; t++) {
+        int o = -1, s = -1;
+        oc::strip_of_column<TX, TY>(t, o, s);
+        if (o < 0 || o >= oc::kOwned || s < 0 || s >= oc::strip_count<TX, TY>(o) ||
+            oc::strip_base<TX, TY>(o) + s != t)
+            return std::printf("column %d: o %d s %d\n", t, o, s), 1;
+    }
     for (int o = 0; o < oc::kOwned; o++) {
         const int dx = oc::own_dx(o), dy = oc::own_dy(o), n = oc::strip_count<TX, TY>(o);
         if (n > oc::strip_max<TX, TY>()) return std::printf("o %d: count %d > max\n", o, n), 1;
@@ -46,8 +64,8 @@ int check() {
 }
 
 int main() {
-    if (check<oc::kTileX, oc::kTileY>() || check<64, 8>() || check<32, 8>() || check<32, 16>() ||
-        check<16, 16>() || check<128, 4>() || check<8, 8>())
+    if (check<oc::kTileX, oc::kTileY>(true) || check<64, 8>(false) || check<32, 8>(false) ||
+        check<32, 16>(false) || check<16, 16>(false) || check<128, 4>(false) || check<8, 8>(false))
         return 1;
     std::printf("ok\n");
     return 0;
