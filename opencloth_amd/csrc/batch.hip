@@ -139,6 +139,106 @@ __device__ inline float assemble_vertex(const DevBatch& c, const StepScalars& s,
     return dot3(r, r);
 }
 
+// Stencil form of assemble_vertex, for batches whose cloths all have the
+// stencil assembly form: the vertex's pairs in stencil order from its mask
+// (bit k: a pair at position k; bit 16 + k: that pair's second spring),
+// neighbour offsets from the position, and each spring's constants from its
+// assembly slot, which is the count of the vertex's springs at earlier
+// positions.  No slot table is read and no load waits on another, so the
+// next position's loads are issued before the current one is evaluated.
+// The pairs, springs and expressions are those of assemble_vertex, in the
+// same order (the slot order is the stencil order on these topologies).
+struct StencilIn {
+    V3 x, v, dv;
+    float L0, ks, kd;
+};
+
+__device__ inline StencilIn fetch_stencil(const DevBatch& c, const StepScalars& s, int i, int nx,
+                                          int k, int slot, bool have) {
+    StencilIn r{};
+    if (have) {
+        const int N = c.Ntot;
+        const int j = i + stencil_dj(k) * nx + stencil_di(k);
+        r.x = ld3(c.x, N, j);
+        r.v = ld3(c.v, N, j);
+        r.dv = s.warm ? ld3(c.dv, N, j) : V3{0.0f, 0.0f, 0.0f};
+        const size_t plane = (size_t)c.wa * N, o = (size_t)slot * N + i;
+        r.L0 = c.slot_c[o];
+        r.ks = c.slot_c[plane + o];
+        r.kd = c.slot_c[2 * plane + o];
+    }
+    return r;
+}
+
+__device__ inline float assemble_vertex_stencil(const DevBatch& c, const StepScalars& s, int i, int nx) {
+    const int N = c.Ntot;
+    const float h = s.h, hh = s.hh;
+    const bool pinned = c.pin[i] != 0;
+    const uint32_t mi = c.asm_mask[i];
+    V3 xi = ld3(c.x, N, i), vi = ld3(c.v, N, i);
+    V3 dvi = s.warm ? ld3(c.dv, N, i) : V3{0.0f, 0.0f, 0.0f};
+    if (!s.warm) st3(c.dv, N, i, dvi);
+    V3 acc = {0.0f, 0.0f, 0.0f}, kv = {0.0f, 0.0f, 0.0f};
+    if (!pinned) acc = {0.0f + s.mass * s.gx, 0.0f + s.mass * s.gy, 0.0f + s.mass * s.gz};
+    if (!pinned && s.damp != 0.0f)
+        acc = {acc.x + s.damp * vi.x, acc.y + s.damp * vi.y, acc.z + s.damp * vi.z};
+    V3 adv = {s.md * dvi.x, s.md * dvi.y, s.md * dvi.z};
+    V3 dg = {s.md, s.md, s.md};
+    const size_t plane = (size_t)c.wa * N;
+    int slot = 0;
+    StencilIn nxt = fetch_stencil(c, s, i, nx, 0, 0, mi & 1u);
+#pragma unroll 1
+    for (int k = 0; k < kStencil; k++) {
+        const StencilIn in = nxt;
+        const bool here = (mi >> k) & 1u;
+        const bool second = (mi >> (16 + k)) & 1u;
+        const int nsp = here ? (second ? 2 : 1) : 0;
+        if (k + 1 < kStencil) nxt = fetch_stencil(c, s, i, nx, k + 1, slot + nsp, (mi >> (k + 1)) & 1u);
+        if (!here) continue;
+        float C[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        float L0 = in.L0, ks = in.ks, kd = in.kd;
+        for (int m = 0; m < nsp; m++) {
+            if (m == 1) {
+                const size_t o = (size_t)(slot + 1) * N + i;
+                L0 = c.slot_c[o];
+                ks = c.slot_c[plane + o];
+                kd = c.slot_c[2 * plane + o];
+            }
+            V3 f;
+            float K[6], P[6];
+            spring_terms(xi, in.x, vi, in.v, L0, ks, kd, f, K, P);
+#pragma unroll
+            for (int q = 0; q < 6; q++) C[q] += h * (kd * P[q]) + hh * K[q];
+            acc = add3(acc, f);
+            kv = add3(kv, symmv(K, sub3(vi, in.v)));
+        }
+        slot += nsp;
+        if (stencil_owner(k)) {
+            const int pl = c.stencil_plane[k];
+#pragma unroll
+            for (int q = 0; q < 6; q++) c.blocks[(size_t)(pl * 6 + q) * N + i] = C[q];
+        }
+        adv = add3(adv, symmv(C, sub3(in.dv, dvi)));
+        dg = {dg.x - C[0], dg.y - C[1], dg.z - C[2]};
+    }
+    V3 b;
+    b.x = pinned ? 0.0f : h * (acc.x + h * kv.x);
+    b.y = pinned ? 0.0f : h * (acc.y + h * kv.y);
+    b.z = pinned ? 0.0f : h * (acc.z + h * kv.z);
+    st3(c.b, N, i, b);
+    V3 r = pinned ? V3{0.0f, 0.0f, 0.0f} : sub3(b, adv);
+    st3(c.r, N, i, r);
+    if (s.flags & OC_FLAG_JACOBI) {
+        const V3 pv = {jacobi_inv(dg.x), jacobi_inv(dg.y), jacobi_inv(dg.z)};
+        st3(c.pinv, N, i, pv);
+        const V3 z = {pv.x * r.x, pv.y * r.y, pv.z * r.z};
+        st3(c.d, N, i, z);
+        return dot3(r, z);
+    }
+    st3(c.d, N, i, r);
+    return dot3(r, r);
+}
+
 // z = P^-1 r of vertex i (r itself without the Jacobi preconditioner).
 __device__ inline V3 precond(const DevBatch& c, const StepScalars& s, int i, V3 r) {
     if (!(s.flags & OC_FLAG_JACOBI)) return r;
@@ -305,7 +405,8 @@ __global__ __launch_bounds__(NT, 4) void k_batch_step(DevBatch c, StepScalars s,
         // ---- assembly and CG start
         float t = 0.0f;
         for (int vv = lt; vv < Ni; vv += T) {
-            const float rr = assemble_vertex(c, s, voff + vv, nx);
+            const float rr = c.stencil_asm ? assemble_vertex_stencil(c, s, voff + vv, nx)
+                                           : assemble_vertex(c, s, voff + vv, nx);
             if constexpr (ADJ)
                 t += adjoint_start_vertex(c, s, adj, voff + vv);
             else

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <numeric>
@@ -260,6 +261,15 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
         for (int k = 0; k < oc::kStencil && stencil; k++)
             stencil = t.stencil_plane[k] == tabs[0].topo.stencil_plane[k];
     }
+    // stencil assembly form (neighbours and slots from a per-vertex mask)
+    // when every cloth has it, unless OPENCLOTH_ASSEMBLY=per-end asks for the
+    // slot-table assembly (measurement)
+    bool st_asm = stencil;
+    for (int c = 0; c < n_cloths && st_asm; c++) st_asm = tabs[c].topo.stencil_asm;
+    {
+        const char* asm_env = std::getenv("OPENCLOTH_ASSEMBLY");
+        if (asm_env && std::strcmp(asm_env, "per-end") == 0) st_asm = false;
+    }
     const size_t N = b->Ntot, S = std::max(b->Stot, 1);
     const size_t vec = align_up(sizeof(float) * 3 * N);
     const size_t blocks = align_up(sizeof(float) * 6 * N * std::max(b->n_planes, 1));
@@ -271,7 +281,8 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
     const size_t slotc = align_up(sizeof(float) * 3 * (size_t)std::max(wa, 1) * N);
     const size_t pin = align_up(N);
     const size_t per = align_up(sizeof(int) * n_cloths);
-    const size_t total = 8 * vec + blocks + as + am + pm + pk + 3 * sp + slotc + pin + 6 * per + 256;
+    const size_t amask = align_up(sizeof(uint32_t) * (st_asm ? N : 1));
+    const size_t total = 8 * vec + blocks + as + am + pm + pk + 3 * sp + slotc + pin + 6 * per + amask + 256;
     if (hipMalloc(&b->arena, total) != hipSuccess) {
         delete b;
         return set_error(OC_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(total) + " bytes failed");
@@ -313,6 +324,8 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
     d.inst_status = (int*)take(per);
     d.inst_sticky = (int*)take(per);
     d.queue = (int*)take(256);
+    d.asm_mask = (const uint32_t*)take(amask);
+    d.stencil_asm = st_asm ? 1 : 0;
 
     auto fail_free = [&](int code, const std::string& msg) {
         (void)hipFree(b->arena);
@@ -327,6 +340,7 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
         std::vector<uint16_t> h_am((size_t)wa * N, oc::kEmptySlot);
         std::vector<uint16_t> h_pm((size_t)wp * N, oc::kEmptySlot);
         std::vector<uint16_t> h_mask(stencil ? N : 0, 0);
+        std::vector<uint32_t> h_amask(st_asm ? N : 0, 0);
         for (int c = 0; c < n_cloths; c++) {
             const oc::GridTopology& t = tabs[c].topo;
             const size_t off = b->voff[c];
@@ -341,12 +355,16 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
                     h_pm[(size_t)k * N + off + v] = t.pair_meta[(size_t)k * t.N + v];
             if (stencil)
                 for (int v = 0; v < t.N; v++) h_mask[off + v] = t.pair_mask[v];
+            if (st_asm)
+                for (int v = 0; v < t.N; v++) h_amask[off + v] = t.asm_mask[v];
         }
         if (hipMemcpy((void*)d.ainc_s, h_as.data(), h_as.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy((void*)d.ainc_m, h_am.data(), h_am.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy((void*)d.pair_m, h_pm.data(), h_pm.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
             (stencil && hipMemcpy((void*)d.pair_mask, h_mask.data(), h_mask.size() * 2,
-                                  hipMemcpyHostToDevice) != hipSuccess))
+                                  hipMemcpyHostToDevice) != hipSuccess) ||
+            (st_asm && hipMemcpy((void*)d.asm_mask, h_amask.data(), h_amask.size() * 4,
+                                 hipMemcpyHostToDevice) != hipSuccess))
             return fail_free(OC_ERR_DEVICE, "upload of slot tables failed");
     }
     tabs.clear();

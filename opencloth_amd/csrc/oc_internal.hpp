@@ -430,6 +430,10 @@ struct DevBatch {
     int* inst_iters;
     int* inst_status;
     int* inst_sticky;  // OR of every step's status bits since the last query
+    // stencil assembly form (every cloth has it): per vertex, bit k a pair at
+    // stencil position k, bit 16 + k that pair's second spring
+    const uint32_t* asm_mask;
+    int stencil_asm;
 };
 
 // Buffers of one batched adjoint step (SoA, stride Ntot).

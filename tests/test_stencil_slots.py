@@ -1,0 +1,50 @@
+"""The batched stencil assembly's slot rule (CPU only): on the reference's
+grid spring lists, a vertex's springs sit in the assembly slots that the
+mask's popcount gives, so batch.hip can read their constants without the
+slot tables.  A small C++ program over the library's topology code checks
+every vertex; a shuffled spring list, which has no stencil form, is
+reported as such."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from opencloth_amd import scenes
+from opencloth_amd.build import ROOT
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def exe(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("slots") / "stencil_slots")
+    csrc = os.path.join(ROOT, "opencloth_amd", "csrc")
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++20", "-O1", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           "-I", os.path.join(ROOT, "include"), "-I", csrc,
+           os.path.join(HERE, "cpp", "stencil_slots.cpp"), os.path.join(csrc, "topology.cpp"), "-o", out]
+    subprocess.run(cmd, check=True, capture_output=True, timeout=300)
+    return out
+
+
+def run(exe, tmp_path, nx, ny, springs):
+    path = str(tmp_path / f"springs_{nx}x{ny}.bin")
+    with open(path, "wb") as f:
+        f.write(np.array([nx, ny, len(springs)], np.int32).tobytes())
+        f.write(springs.tobytes())
+    r = subprocess.run([exe, path], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout.strip()
+
+
+@pytest.mark.parametrize("nx,ny", [(3, 3), (3, 5), (16, 16), (21, 21), (37, 90), (128, 17)])
+def test_slot_rule_on_grid_springs(exe, tmp_path, nx, ny):
+    X = scenes.grid_positions(nx, ny)
+    assert run(exe, tmp_path, nx, ny, scenes.grid_springs(nx, ny, X)) == "ok"
+
+
+def test_shuffled_list_has_no_stencil_form(exe, tmp_path):
+    X = scenes.grid_positions(21, 21)
+    sp = scenes.grid_springs(21, 21, X)
+    sp = sp[np.random.default_rng(3).permutation(len(sp))]
+    assert run(exe, tmp_path, 21, 21, sp) in ("ok", "no stencil assembly form")
