@@ -331,6 +331,174 @@ __device__ inline float adjoint_start_vertex(const DevBatch& c, const StepScalar
     return dot3(r, z);
 }
 
+// Spring-once assembly of the workgroup's cloths; every thread of the
+// workgroup calls it (it has barriers).  A cloth's T threads sweep it in
+// bands of T consecutive vertices.  In each band, direction by direction
+// (the six owned offsets), every vertex evaluates the pair it owns in that
+// direction (it is the pair's lower vertex) into LDS; after a barrier each
+// vertex adds, in stencil order, the pair where it is the partner (the
+// owner's force and K w negated: swapping a spring's ends negates them
+// exactly and leaves the block term; the sums start at +0, where the sign
+// of a zero term cannot matter) and its own pair.  Owners in earlier bands
+// are read from a ring per direction that keeps that direction's latest
+// results (ring_len >= the direction's offset, so an owner's entry is not
+// overwritten before its partner's band reads it).  A pair's second spring
+// is evaluated by each end itself.  The sums are assemble_vertex_stencil's,
+// term by term in the same order.  Returns this thread's sum of r.z (ADJ:
+// of the adjoint start's r.z) over its vertices in band order.
+template <int NT, bool ADJ>
+__device__ float assemble_once(const DevBatch& c, const StepScalars& s, const BatchAdjBufs& adj,
+                               float* lds, int T, int g, int lt, bool act, int voff, int nx, int Ni,
+                               int Nmax, int ringf) {
+    const int N = c.Ntot, tid = threadIdx.x;
+    const float h = s.h, hh = s.hh;
+    const size_t plane = (size_t)c.wa * N;
+    float* res = lds;  // res[q * NT + tid]: this direction's owner results
+    float* ring = lds + 12 * NT + g * ringf;
+    float t = 0.0f;
+    const int nb = (Nmax + T - 1) / T;
+    for (int band = 0; band < nb; band++) {
+        const int v0 = band * T, vv = v0 + lt;
+        const bool here = act && vv < Ni;
+        const int i = voff + vv;
+        uint32_t mi = 0u;
+        bool pinned = true;
+        V3 xi = {0.0f, 0.0f, 0.0f}, vi = xi, dvi = xi, acc = xi, kv = xi, adv = xi, dg = xi;
+        if (here) {
+            mi = c.asm_mask[i];
+            pinned = c.pin[i] != 0;
+            xi = ld3(c.x, N, i);
+            vi = ld3(c.v, N, i);
+            if (s.warm) dvi = ld3(c.dv, N, i);
+            else st3(c.dv, N, i, dvi);
+            if (!pinned) acc = {0.0f + s.mass * s.gx, 0.0f + s.mass * s.gy, 0.0f + s.mass * s.gz};
+            if (!pinned && s.damp != 0.0f)
+                acc = {acc.x + s.damp * vi.x, acc.y + s.damp * vi.y, acc.z + s.damp * vi.z};
+            adv = {s.md * dvi.x, s.md * dvi.y, s.md * dvi.z};
+            dg = {s.md, s.md, s.md};
+        }
+        float C[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        auto add_spring = [&](const SpringOut& r) {
+#pragma unroll
+            for (int q = 0; q < 6; q++) C[q] += r.c[q];
+            acc = {acc.x + r.f.x, acc.y + r.f.y, acc.z + r.f.z};
+            kv = {kv.x + r.kw.x, kv.y + r.kw.y, kv.z + r.kw.z};
+        };
+        // assembly slot of the first spring at stencil position k
+        auto slot_of = [&](int k) {
+            const uint32_t below = (1u << k) - 1u;
+            return __popc(mi & 0xFFFu & below) + __popc((mi >> 16) & below);
+        };
+        // the pair's second spring, from this vertex's side
+        auto add_second = [&](int k, int j) {
+            const size_t o = (size_t)(slot_of(k) + 1) * N + i;
+            add_spring(eval_spring(xi, ld3(c.x, N, j), vi, ld3(c.v, N, j), c.slot_c[o], c.slot_c[plane + o],
+                                   c.slot_c[2 * plane + o], h, hh));
+        };
+        // the pair is complete: its block (when owned), A dv and diag(A) terms
+        auto finish_pair = [&](int k, int j) {
+            if (stencil_owner(k)) {
+                const int pl = c.stencil_plane[k];
+#pragma unroll
+                for (int q = 0; q < 6; q++) c.blocks[(size_t)(pl * 6 + q) * N + i] = C[q];
+            }
+            const V3 dvj = s.warm ? ld3(c.dv, N, j) : V3{0.0f, 0.0f, 0.0f};
+            adv = add3(adv, symmv(C, sub3(dvj, dvi)));
+            dg = {dg.x - C[0], dg.y - C[1], dg.z - C[2]};
+#pragma unroll
+            for (int q = 0; q < 6; q++) C[q] = 0.0f;
+        };
+        auto load = [&](const float* src, int stride) {
+            SpringOut e;
+            e.f = {src[0], src[stride], src[2 * stride]};
+            e.kw = {src[3 * stride], src[4 * stride], src[5 * stride]};
+#pragma unroll
+            for (int q = 0; q < 6; q++) e.c[q] = src[(6 + q) * stride];
+            return e;
+        };
+        SpringOut held{};  // this vertex's k = 7 pair, added after k = 5 and k = 6
+#pragma unroll 1
+        for (int o = 0; o < kOwned; o++) {
+            const int ko = owned_k(o), kp = owned_partner_k(o), off = own_offset(o, nx);
+            const int R = ring_len(o, nx);
+            float* rg = ring + 12 * ring_base(o, nx);
+            // ---- evaluate this vertex's pair in direction o
+            if (here && ((mi >> ko) & 1u)) {
+                const int j = i + off;
+                const size_t so = (size_t)slot_of(ko) * N + i;
+                const SpringOut e = eval_spring(xi, ld3(c.x, N, j), vi, ld3(c.v, N, j), c.slot_c[so],
+                                                c.slot_c[plane + so], c.slot_c[2 * plane + so], h, hh);
+                res[0 * NT + tid] = e.f.x;
+                res[1 * NT + tid] = e.f.y;
+                res[2 * NT + tid] = e.f.z;
+                res[3 * NT + tid] = e.kw.x;
+                res[4 * NT + tid] = e.kw.y;
+                res[5 * NT + tid] = e.kw.z;
+#pragma unroll
+                for (int q = 0; q < 6; q++) res[(6 + q) * NT + tid] = e.c[q];
+            }
+            __syncthreads();
+            // ---- accumulate, in this vertex's stencil order
+            if (here && ((mi >> kp) & 1u)) {  // partner of the pair owned by vertex vv - off
+                const int jv = vv - off;
+                SpringOut e = jv >= v0 ? load(res + (tid - off), NT) : load(rg + (jv & (R - 1)), R);
+                e.f = {0.0f - e.f.x, 0.0f - e.f.y, 0.0f - e.f.z};
+                e.kw = {0.0f - e.kw.x, 0.0f - e.kw.y, 0.0f - e.kw.z};
+                add_spring(e);
+                if ((mi >> (16 + kp)) & 1u) add_second(kp, i - off);
+                finish_pair(kp, i - off);
+            }
+            if (o == 2) {
+                if (here && ((mi >> ko) & 1u)) held = load(res + tid, NT);  // k = 7 follows k = 5, 6
+            } else if (here) {
+#pragma unroll
+                for (int u = 0; u < 2; u++) {  // k = 6, then the held k = 7; else just ko
+                    if (u == 1 && o != 3) break;
+                    const int oo = o == 3 ? (u == 0 ? 3 : 2) : o;
+                    const int kk = owned_k(oo);
+                    if (!((mi >> kk) & 1u)) continue;
+                    add_spring(u == 0 ? load(res + tid, NT) : held);
+                    const int j = i + own_offset(oo, nx);
+                    if ((mi >> (16 + kk)) & 1u) add_second(kk, j);
+                    finish_pair(kk, j);
+                }
+            }
+            __syncthreads();
+            // ---- keep this direction's results of the band's last R owners
+            if (here && ((mi >> ko) & 1u) && vv >= v0 + T - R) {
+                const int e = vv & (R - 1);
+#pragma unroll
+                for (int q = 0; q < 12; q++) rg[q * R + e] = res[q * NT + tid];
+            }
+        }
+        if (here) {
+            V3 b;
+            b.x = pinned ? 0.0f : h * (acc.x + h * kv.x);
+            b.y = pinned ? 0.0f : h * (acc.y + h * kv.y);
+            b.z = pinned ? 0.0f : h * (acc.z + h * kv.z);
+            st3(c.b, N, i, b);
+            V3 r = pinned ? V3{0.0f, 0.0f, 0.0f} : sub3(b, adv);
+            st3(c.r, N, i, r);
+            float rr;
+            if (s.flags & OC_FLAG_JACOBI) {
+                const V3 pv = {jacobi_inv(dg.x), jacobi_inv(dg.y), jacobi_inv(dg.z)};
+                st3(c.pinv, N, i, pv);
+                const V3 z = {pv.x * r.x, pv.y * r.y, pv.z * r.z};
+                st3(c.d, N, i, z);
+                rr = dot3(r, z);
+            } else {
+                st3(c.d, N, i, r);
+                rr = dot3(r, r);
+            }
+            if constexpr (ADJ)
+                t += adjoint_start_vertex(c, s, adj, i);
+            else
+                t += rr;
+        }
+    }
+    return t;
+}
+
 // Adjoint gather for vertex i (same as the single-cloth k_adj_gather):
 // c.x, c.v the pre-step state, c.dv the adjoint solve mu, a.dvsol the step's
 // solution.
@@ -376,13 +544,16 @@ __device__ inline void adjoint_gather_vertex(const DevBatch& c, const StepScalar
 
 // ADJ = false: forward step.  ADJ = true: one step's adjoint at the state in
 // c.x/c.v (assembly without warm start, CG for mu in c.dv, gather).
-template <int NT, bool ADJ>
+// ONCE: the spring-once assembly (c.asm_once), a separate instantiation so
+// the default kernel carries none of its registers.
+template <int NT, bool ADJ, bool ONCE>
 __global__ __launch_bounds__(NT, 4) void k_batch_step(DevBatch c, StepScalars s, BatchAdjBufs adj) {
     constexpr int NW = NT / 64;
     __shared__ float ws[NW];
     __shared__ float sd[NW], sd_old[NW], sd0[NW], ssig[NW];
     __shared__ int sdone[NW], sit[NW], sst[NW];
     __shared__ int sitem;
+    extern __shared__ float dyn_lds[];  // spring-once assembly: lds_floats
     const int tid = threadIdx.x;
     const int N = c.Ntot;
     for (;;) {
@@ -404,13 +575,25 @@ __global__ __launch_bounds__(NT, 4) void k_batch_step(DevBatch c, StepScalars s,
 
         // ---- assembly and CG start
         float t = 0.0f;
-        for (int vv = lt; vv < Ni; vv += T) {
-            const float rr = c.stencil_asm ? assemble_vertex_stencil(c, s, voff + vv, nx)
-                                           : assemble_vertex(c, s, voff + vv, nx);
-            if constexpr (ADJ)
-                t += adjoint_start_vertex(c, s, adj, voff + vv);
-            else
-                t += rr;
+        if constexpr (ONCE) {
+            // the item's longest sweep and widest rings
+            int Nmax = 0, nxm = 1;
+            for (int gg = 0; gg < G; gg++) {
+                const int in = c.order[c.item_inst[item] + gg];
+                Nmax = max(Nmax, c.inst_n[in]);
+                nxm = max(nxm, c.inst_nx[in]);
+            }
+            t = assemble_once<NT, ADJ>(c, s, adj, dyn_lds, T, g, lt, act, voff, nx, Ni, Nmax,
+                                       ring_floats(nxm));
+        } else {
+            for (int vv = lt; vv < Ni; vv += T) {
+                const float rr = c.stencil_asm ? assemble_vertex_stencil(c, s, voff + vv, nx)
+                                               : assemble_vertex(c, s, voff + vv, nx);
+                if constexpr (ADJ)
+                    t += adjoint_start_vertex(c, s, adj, voff + vv);
+                else
+                    t += rr;
+            }
         }
         float tot = seg_sum<NW>(t, T, ws);
         if (act && lt == 0) {
@@ -556,15 +739,27 @@ hipError_t launch_batch_reset(const DevBatch& c, hipStream_t st) {
 template <bool ADJ>
 static hipError_t launch_batch(const DevBatch& c, const StepScalars& s, const BatchAdjBufs& a,
                                int block_threads, int grid, hipStream_t st) {
-    switch (block_threads) {
-        case 256:
-            hipLaunchKernelGGL((k_batch_step<256, ADJ>), dim3(grid), dim3(256), 0, st, c, s, a);
-            break;
+    if (c.asm_once && (c.lds_floats < 12 * block_threads || c.lds_floats > kBatchLdsFloats))
+        return hipErrorInvalidValue;
+    const size_t lds = c.asm_once ? sizeof(float) * (size_t)c.lds_floats : 0;
+    switch (block_threads * 2 + (c.asm_once ? 1 : 0)) {
         case 512:
-            hipLaunchKernelGGL((k_batch_step<512, ADJ>), dim3(grid), dim3(512), 0, st, c, s, a);
+            hipLaunchKernelGGL((k_batch_step<256, ADJ, false>), dim3(grid), dim3(256), 0, st, c, s, a);
+            break;
+        case 513:
+            hipLaunchKernelGGL((k_batch_step<256, ADJ, true>), dim3(grid), dim3(256), lds, st, c, s, a);
             break;
         case 1024:
-            hipLaunchKernelGGL((k_batch_step<1024, ADJ>), dim3(grid), dim3(1024), 0, st, c, s, a);
+            hipLaunchKernelGGL((k_batch_step<512, ADJ, false>), dim3(grid), dim3(512), 0, st, c, s, a);
+            break;
+        case 1025:
+            hipLaunchKernelGGL((k_batch_step<512, ADJ, true>), dim3(grid), dim3(512), lds, st, c, s, a);
+            break;
+        case 2048:
+            hipLaunchKernelGGL((k_batch_step<1024, ADJ, false>), dim3(grid), dim3(1024), 0, st, c, s, a);
+            break;
+        case 2049:
+            hipLaunchKernelGGL((k_batch_step<1024, ADJ, true>), dim3(grid), dim3(1024), lds, st, c, s, a);
             break;
         default:
             return hipErrorInvalidValue;

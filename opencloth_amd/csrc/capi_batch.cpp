@@ -46,6 +46,10 @@ struct oc_batch_s {
     size_t adj_bytes = 0;
     int roll_steps = 0, roll_every = 0;
     bool roll_valid = false;
+    // spring-once batched assembly (OPENCLOTH_ASSEMBLY=once) where the
+    // schedule's LDS fits; off by default: on the 4096-cloth workload it ran
+    // 9.30 ms per step against 8.45 ms for the per-end stencil form
+    bool allow_once = false;
 };
 
 namespace {
@@ -110,6 +114,20 @@ int apply_schedule(oc_batch_s* b) {
         k += cnt;
     }
     const int n_items = (int)it_inst.size();
+    // spring-once assembly: 12 floats per thread plus each cloth's rings
+    // (sized for the item's widest cloth), when that fits the LDS budget
+    int lds_floats = 12 * NT;
+    for (int k = 0; k < n_items; k++) {
+        int nxm = 1;
+        for (int q = 0; q < it_count[k]; q++) nxm = std::max(nxm, b->nx[order[it_inst[k] + q]]);
+        lds_floats = std::max(lds_floats, 12 * NT + it_count[k] * oc::ring_floats(nxm));
+    }
+    int dev_lds = 0, cur = 0;
+    OC_HIP(hipGetDevice(&cur));
+    OC_HIP(hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, cur));
+    // the kernel's static LDS (a few hundred bytes) comes on top
+    const int lds_cap = std::min(oc::kBatchLdsFloats, (dev_lds - 1024) / 4);
+    const bool once = b->dc.stencil_asm && b->allow_once && lds_floats <= lds_cap;
     // persistent grid: as many workgroups as fit on the device at once
     int dev = 0, cus = 0;
     OC_HIP(hipGetDevice(&dev));
@@ -145,6 +163,8 @@ int apply_schedule(oc_batch_s* b) {
     b->dc.item_T = d_T;
     b->dc.order = d_order;
     b->dc.n_items = n_items;
+    b->dc.asm_once = once ? 1 : 0;
+    b->dc.lds_floats = once ? lds_floats : 0;
     b->grid = grid;
     b->graph_dirty = true;
     return OC_OK;
@@ -266,9 +286,10 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
     // slot-table assembly (measurement)
     bool st_asm = stencil;
     for (int c = 0; c < n_cloths && st_asm; c++) st_asm = tabs[c].topo.stencil_asm;
-    {
+    {  // OPENCLOTH_ASSEMBLY=once asks for the spring-once form
         const char* asm_env = std::getenv("OPENCLOTH_ASSEMBLY");
         if (asm_env && std::strcmp(asm_env, "per-end") == 0) st_asm = false;
+        if (asm_env && std::strcmp(asm_env, "once") == 0) b->allow_once = true;
     }
     const size_t N = b->Ntot, S = std::max(b->Stot, 1);
     const size_t vec = align_up(sizeof(float) * 3 * N);

@@ -321,22 +321,6 @@ __global__ __launch_bounds__(TX* TY) void k_assemble(DevCloth c, StepScalars s) 
 // zero cannot change a sum; so the results are the per-end kernel's bits.
 // A second spring of a pair (the reference's duplicated last bends) is
 // evaluated by each end itself.
-// One spring from the x1 end: force on x1, K (v1 - v2) and the pair-block
-// term h (kd P) + h^2 K, in the per-end kernel's expressions.
-struct SpringOut {
-    V3 f, kw;
-    float c[6];
-};
-__device__ inline SpringOut eval_spring(V3 x1, V3 x2, V3 v1, V3 v2, float L0, float ks, float kd,
-                                        float h, float hh) {
-    SpringOut o;
-    float K[6], P[6];
-    spring_terms(x1, x2, v1, v2, L0, ks, kd, o.f, K, P);
-#pragma unroll
-    for (int q = 0; q < 6; q++) o.c[q] = h * (kd * P[q]) + hh * K[q];
-    o.kw = symmv(K, V3{v1.x - v2.x, v1.y - v2.y, v1.z - v2.z});
-    return o;
-}
 
 template <int TX, int TY>
 struct AssembleStShared {

@@ -89,6 +89,23 @@ __device__ inline void spring_terms(V3 x1, V3 x2, V3 v1, V3 v2, float L0, float 
     f = {s * (dp.x * inv), s * (dp.y * inv), s * (dp.z * inv)};
 }
 
+// One spring from the x1 end: force on x1, K (v1 - v2) and the pair-block
+// term h (kd P) + h^2 K, in the per-end assembly's expressions.
+struct SpringOut {
+    V3 f, kw;
+    float c[6];
+};
+__device__ inline SpringOut eval_spring(V3 x1, V3 x2, V3 v1, V3 v2, float L0, float ks, float kd,
+                                        float h, float hh) {
+    SpringOut o;
+    float K[6], P[6];
+    spring_terms(x1, x2, v1, v2, L0, ks, kd, o.f, K, P);
+#pragma unroll
+    for (int q = 0; q < 6; q++) o.c[q] = h * (kd * P[q]) + hh * K[q];
+    o.kw = symmv(K, V3{v1.x - v2.x, v1.y - v2.y, v1.z - v2.z});
+    return o;
+}
+
 // Force only (no Jacobian), same arithmetic as spring_terms.
 __device__ inline V3 spring_force(V3 x1, V3 x2, V3 v1, V3 v2, float L0, float Ks, float Kd) {
     V3 dp = {x1.x - x2.x, x1.y - x2.y, x1.z - x2.z};

@@ -434,7 +434,35 @@ struct DevBatch {
     // stencil position k, bit 16 + k that pair's second spring
     const uint32_t* asm_mask;
     int stencil_asm;
+    // spring-once assembly (stencil assembly form and the schedule's LDS
+    // fits): each pair evaluated once by its lower vertex, the partner reads
+    // the result from LDS; lds_floats of dynamic LDS per workgroup
+    int asm_once;
+    int lds_floats;
 };
+
+// Spring-once batched assembly: owner results of direction o for the
+// vertices before the current band live in a ring of ring_len(o, nx)
+// entries (a power of two >= the direction's index offset); a cloth's rings
+// take ring_floats(nx) floats (12 per entry).
+__host__ __device__ constexpr int own_offset(int o, int nx) { return own_dy(o) * nx + own_dx(o); }
+__host__ __device__ inline int pow2_at_least(int x) {
+    int p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+__host__ __device__ inline int ring_len(int o, int nx) { return pow2_at_least(own_offset(o, nx)); }
+__host__ __device__ inline int ring_base(int o, int nx) {  // in entries
+    int b = 0;
+    for (int q = 0; q < o; q++) b += ring_len(q, nx);
+    return b;
+}
+__host__ __device__ inline int ring_floats(int nx) { return 12 * ring_base(kOwned, nx); }
+// Dynamic LDS of the spring-once batched assembly: 12 floats per thread for
+// the current direction's results, then ring_floats of the widest cloth per
+// cloth of the work item; schedules needing more than the device allows per
+// workgroup (at most 160 KiB) keep the per-vertex form.
+constexpr int kBatchLdsFloats = 160 * 1024 / 4;
 
 // Buffers of one batched adjoint step (SoA, stride Ntot).
 struct BatchAdjBufs {

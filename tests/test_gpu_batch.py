@@ -160,6 +160,35 @@ def test_batch_graph_equals_eager():
     assert np.array_equal(Va.view(np.uint32), Vb.view(np.uint32))
 
 
+@pytest.mark.parametrize("sched", [(256, 1), (512, 1), (1024, 1), (512, 32)])
+def test_batch_spring_once_form_bitwise(monkeypatch, sched):
+    """The opt-in spring-once batched assembly (OPENCLOTH_ASSEMBLY=once: each
+    pair evaluated by its lower vertex, the partner reading the result from
+    LDS, owners of earlier bands from rings) equals the default form bit for
+    bit: 3 steps and the state, on cloths up to 128 wide, including
+    schedules whose threads per cloth are fewer than two grid rows."""
+    from opencloth_amd.batch import ClothBatch
+
+    sc = small_batch(n=12, seed=9, sides=(16, 128))
+    say("spring-once batch form, schedule", sched)
+    ref = ClothBatch.from_scene(sc)
+    monkeypatch.setenv("OPENCLOTH_ASSEMBLY", "once")
+    once = ClothBatch.from_scene(sc)
+    monkeypatch.delenv("OPENCLOTH_ASSEMBLY")
+    for b in (ref, once):
+        b.set_schedule(*sched)
+        b.step(3, graph=False)
+    Xa, Va = ref.get_state()
+    Xb, Vb = once.get_state()
+    ita, sta = ref.status()
+    itb, stb = once.status()
+    ref.close()
+    once.close()
+    assert np.array_equal(ita, itb) and np.array_equal(sta, stb)
+    assert np.array_equal(Xa.view(np.uint32), Xb.view(np.uint32))
+    assert np.array_equal(Va.view(np.uint32), Vb.view(np.uint32))
+
+
 def test_batch_close_to_single_cloth_path():
     from opencloth_amd.batch import ClothBatch
     from opencloth_amd.cloth import ImplicitCloth

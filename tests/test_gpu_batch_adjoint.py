@@ -51,6 +51,20 @@ def test_batch_adjoint_bitwise_against_oracle(sched):
         assert np.array_equal(gx0[v0:v0 + n].view(np.uint32), rx0.view(np.uint32)), c
 
 
+def test_batch_adjoint_spring_once_form_bitwise(monkeypatch):
+    """The opt-in spring-once batched assembly (OPENCLOTH_ASSEMBLY=once) in
+    the forward rollout and the adjoint: final state and all gradients equal
+    the default form's bit for bit."""
+    sc = scenes.batch_diff_scene(n_cloths=8, sides=(16, 70), steps=7, seed=4)
+    say("batch adjoint, spring-once form")
+    ref = run_batch(sc, 7, 3, (512, 1))
+    monkeypatch.setenv("OPENCLOTH_ASSEMBLY", "once")
+    once = run_batch(sc, 7, 3, (512, 1))
+    monkeypatch.delenv("OPENCLOTH_ASSEMBLY")
+    for a, b in zip(ref[1:], once[1:]):
+        assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32))
+
+
 def test_batch_adjoint_against_fp64_sample():
     """A smaller instance of the batched differentiable workload: FP32 batch
     gradients of 4 sampled cloths against the FP64 oracle adjoint."""

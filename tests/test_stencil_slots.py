@@ -48,3 +48,31 @@ def test_shuffled_list_has_no_stencil_form(exe, tmp_path):
     sp = scenes.grid_springs(21, 21, X)
     sp = sp[np.random.default_rng(3).permutation(len(sp))]
     assert run(exe, tmp_path, 21, 21, sp) in ("ok", "no stencil assembly form")
+
+
+@pytest.fixture(scope="module")
+def ring_exe(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("ring") / "band_ring")
+    csrc = os.path.join(ROOT, "opencloth_amd", "csrc")
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O1", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           "-I", os.path.join(ROOT, "include"), "-I", csrc,
+           os.path.join(HERE, "cpp", "band_ring.cpp"), os.path.join(csrc, "topology.cpp"), "-o", out]
+    subprocess.run(cmd, check=True, capture_output=True, timeout=300)
+    return out
+
+
+@pytest.mark.parametrize("nx,ny,T,NT", [(16, 16, 256, 512), (16, 16, 64, 512), (21, 21, 64, 256),
+                                        (37, 90, 512, 512), (128, 128, 512, 512), (128, 17, 64, 512),
+                                        (100, 33, 1024, 1024), (3, 5, 64, 1024)])
+def test_band_rings(ring_exe, tmp_path, nx, ny, T, NT):
+    """The batched spring-once assembly's bands and rings (CPU walk-through):
+    every partner reads its own owner's result, inside the LDS allocation,
+    also where T is shorter than a ring's offset (two grid rows)."""
+    X = scenes.grid_positions(nx, ny)
+    sp = scenes.grid_springs(nx, ny, X)
+    path = str(tmp_path / "springs.bin")
+    with open(path, "wb") as f:
+        f.write(np.array([nx, ny, len(sp)], np.int32).tobytes())
+        f.write(sp.tobytes())
+    r = subprocess.run([ring_exe, path, str(T), str(NT)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr
