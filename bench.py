@@ -248,7 +248,11 @@ def bench_batch(steps: int, warmup: int, n_cloths: int = 4096, plain_steps: int 
     t1 = time.perf_counter()
     b = ClothBatch.from_scene(sc)
     t2 = time.perf_counter()
+    # the schedule is tuned on the state two steps in (where the timed steps
+    # run), then the cloths go back to rest
+    b.step(2)
     sched = b.autotune()
+    b.set_state(sc.X, sc.V)
     b.step(warmup)
     b.synchronize()
     e0, e1 = Event(), Event()
@@ -257,12 +261,21 @@ def bench_batch(steps: int, warmup: int, n_cloths: int = 4096, plain_steps: int 
     e1.record()
     e1.synchronize()
     ms = e0.elapsed_ms(e1)
+    # the roofline launch: one more step timed alone; its algorithmic bytes
+    # come from its own CG iteration counts (the committed counters are of
+    # this launch: the batch leg's last k_batch_step dispatch)
+    r0, r1 = Event(), Event()
+    r0.record()
+    b.step(1)
+    r1.record()
+    r1.synchronize()
+    launch_s = r0.elapsed_ms(r1) * 1e-3
     it, st = b.status()
     b.close()
     N = int(sc.X.shape[0])
-    # algorithmic bytes of the last step: the single-cloth model per cloth with
-    # its own CG iteration count (P = S - (nx + ny): one duplicate bend per
-    # grid row and column)
+    # algorithmic bytes of the roofline launch: the single-cloth model per
+    # cloth with its own CG iteration count (P = S - (nx + ny): one duplicate
+    # bend per grid row and column)
     nxs, nys, ss = sc.nx.astype(np.int64), sc.ny.astype(np.int64), sc.n_springs.astype(np.int64)
     ns = nxs * nys
     ps = ss - (nxs + nys)
@@ -297,7 +310,6 @@ def bench_batch(steps: int, warmup: int, n_cloths: int = 4096, plain_steps: int 
     ms_plain_p = plain_form(True) if plain else float("nan")
     traffic = read_profile_value(os.path.join(ROOT, "profiles", "batch_step_4096_cloths_pmc.txt"),
                                  "batch_step_hbm_bytes_per_launch")
-    step_s = ms / steps * 1e-3
     return {
         "workload": f"{sc.n_cloths} cloths, sides U[16,128], {N} vertices, {len(sc.springs)} springs, "
                     "CG <= 40 it, rel tol 1e-4, dt 1/60",
@@ -308,15 +320,15 @@ def bench_batch(steps: int, warmup: int, n_cloths: int = 4096, plain_steps: int 
         "cg_iters_mean": float(it.mean()), "cg_iters_max": int(it.max()),
         "status_nonzero": int((st != 0).sum()),
         "step_algorithmic_bytes": step_b,
-        "step_roofline_frac": step_b / (ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "step_roofline_frac": step_b / launch_s / 1e9 / HBM_PEAK_GBS,
         "roofline": {
-            "bound": "hbm", "kernel": "k_batch_step (forward)",
-            "achieved": step_b / step_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": step_b / step_s / 1e9 / HBM_PEAK_GBS,
-            "algorithmic_bytes_per_launch": step_b, "avg_launch_us": step_s * 1e6,
+            "bound": "hbm", "kernel": "k_batch_step (forward), the step after the timed steps",
+            "achieved": step_b / launch_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": step_b / launch_s / 1e9 / HBM_PEAK_GBS,
+            "algorithmic_bytes_per_launch": step_b, "avg_launch_us": launch_s * 1e6,
             "traffic": traffic,
-            "traffic_GBps": None if traffic is None else traffic / step_s / 1e9,
-            "traffic_frac": None if traffic is None else traffic / step_s / 1e9 / HBM_PEAK_GBS,
+            "traffic_GBps": None if traffic is None else traffic / launch_s / 1e9,
+            "traffic_frac": None if traffic is None else traffic / launch_s / 1e9 / HBM_PEAK_GBS,
         },
         "plain_ms_per_step": ms_plain,
         "plain_vertex_steps_per_s": N / (ms_plain * 1e-3),
