@@ -475,7 +475,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched legs")
     ap.add_argument("--no-entries", action="store_true",
                     help="skip the Verlet / PBD / StepPhysics timings")
-    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the batched legs' one-cloth-per-launch denominators (profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
