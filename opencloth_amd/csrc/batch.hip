@@ -141,20 +141,23 @@ __device__ inline float assemble_vertex(const DevBatch& c, const StepScalars& s,
 
 // Stencil form of assemble_vertex, for batches whose cloths all have the
 // stencil assembly form: the vertex's pairs in stencil order from its mask
-// (bit k: a pair at position k; bit 16 + k: that pair's second spring),
-// neighbour offsets from the position, and each spring's constants from its
-// assembly slot, which is the count of the vertex's springs at earlier
-// positions.  No slot table is read and no load waits on another, so the
-// next position's loads are issued before the current one is evaluated.
-// The pairs, springs and expressions are those of assemble_vertex, in the
-// same order (the slot order is the stencil order on these topologies).
+// (bit k: a pair at position k; bit 16 + k: that pair's second spring) and
+// neighbour offsets from the position.  A pair's first spring takes its
+// constants from the owner planes (c.cst) at the pair's lower vertex, so the
+// partner's read hits the cache line its owner's thread loads; a second
+// spring takes them from its assembly slot, which is the count of the
+// vertex's springs at earlier positions, plus one.  No slot table is read
+// and no load waits on another, so the next position's loads are issued
+// before the current one is evaluated.  The pairs, springs and expressions
+// are those of assemble_vertex, in the same order (the slot order is the
+// stencil order on these topologies).
 struct StencilIn {
     V3 x, v, dv;
     float L0, ks, kd;
 };
 
 __device__ inline StencilIn fetch_stencil(const DevBatch& c, const StepScalars& s, int i, int nx,
-                                          int k, int slot, bool have) {
+                                          int k, bool have) {
     StencilIn r{};
     if (have) {
         const int N = c.Ntot;
@@ -162,10 +165,10 @@ __device__ inline StencilIn fetch_stencil(const DevBatch& c, const StepScalars& 
         r.x = ld3(c.x, N, j);
         r.v = ld3(c.v, N, j);
         r.dv = s.warm ? ld3(c.dv, N, j) : V3{0.0f, 0.0f, 0.0f};
-        const size_t plane = (size_t)c.wa * N, o = (size_t)slot * N + i;
-        r.L0 = c.slot_c[o];
-        r.ks = c.slot_c[plane + o];
-        r.kd = c.slot_c[2 * plane + o];
+        const size_t o = (size_t)owned_index(k) * 3 * N + (stencil_owner(k) ? i : j);
+        r.L0 = c.cst[o];
+        r.ks = c.cst[o + N];
+        r.kd = c.cst[o + 2 * (size_t)N];
     }
     return r;
 }
@@ -186,14 +189,14 @@ __device__ inline float assemble_vertex_stencil(const DevBatch& c, const StepSca
     V3 dg = {s.md, s.md, s.md};
     const size_t plane = (size_t)c.wa * N;
     int slot = 0;
-    StencilIn nxt = fetch_stencil(c, s, i, nx, 0, 0, mi & 1u);
+    StencilIn nxt = fetch_stencil(c, s, i, nx, 0, mi & 1u);
 #pragma unroll 1
     for (int k = 0; k < kStencil; k++) {
         const StencilIn in = nxt;
         const bool here = (mi >> k) & 1u;
         const bool second = (mi >> (16 + k)) & 1u;
         const int nsp = here ? (second ? 2 : 1) : 0;
-        if (k + 1 < kStencil) nxt = fetch_stencil(c, s, i, nx, k + 1, slot + nsp, (mi >> (k + 1)) & 1u);
+        if (k + 1 < kStencil) nxt = fetch_stencil(c, s, i, nx, k + 1, (mi >> (k + 1)) & 1u);
         if (!here) continue;
         float C[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         float L0 = in.L0, ks = in.ks, kd = in.kd;
@@ -710,6 +713,31 @@ __global__ __launch_bounds__(NT, 4) void k_batch_step(DevBatch c, StepScalars s,
     }
 }
 
+// Owner-plane constants from the slot-major copies: the first spring of the
+// pair vertex i owns in direction o sits at its assembly slot (the count of
+// the vertex's springs at earlier stencil positions).
+__global__ __launch_bounds__(256) void k_owner_constants(DevBatch c) {
+    const int N = c.Ntot;
+    const size_t plane = (size_t)c.wa * N;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
+        const uint32_t mi = c.asm_mask[i];
+#pragma unroll
+        for (int o = 0; o < kOwned; o++) {
+            const int k = owned_k(o);
+            const uint32_t below = (1u << k) - 1u;
+            const bool own = (mi >> k) & 1u;
+            // (a vertex with this pair has a spring at that slot, so slot < wa)
+            const int slot = min(__popc(mi & 0xFFFu & below) + __popc((mi >> 16) & below), c.wa - 1);
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                float v = 0.0f;
+                if (own) v = c.slot_c[q * plane + (size_t)slot * N + i];
+                c.cst[(size_t)(o * 3 + q) * N + i] = v;
+            }
+        }
+    }
+}
+
 __global__ void k_batch_reset(int* queue) {
     if (threadIdx.x == 0) *queue = 0;
 }
@@ -728,6 +756,12 @@ __global__ __launch_bounds__(256) void k_slot_constants(DevBatch c, int q0, int 
 hipError_t launch_batch_slot_constants(const DevBatch& c, int q0, int nq, hipStream_t st) {
     if (q0 < 0 || nq < 0 || q0 + nq > 3) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_slot_constants, dim3(4096), dim3(256), 0, st, c, q0, nq);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_owner_constants(const DevBatch& c, hipStream_t st) {
+    if (!c.stencil_asm) return hipSuccess;
+    hipLaunchKernelGGL(k_owner_constants, dim3(4096), dim3(256), 0, st, c);
     return hipGetLastError();
 }
 

@@ -303,7 +303,8 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
     const size_t pin = align_up(N);
     const size_t per = align_up(sizeof(int) * n_cloths);
     const size_t amask = align_up(sizeof(uint32_t) * (st_asm ? N : 1));
-    const size_t total = 8 * vec + blocks + as + am + pm + pk + 3 * sp + slotc + pin + 6 * per + amask + 256;
+    const size_t cst = align_up(sizeof(float) * 3 * oc::kOwned * (st_asm ? N : 1));
+    const size_t total = 8 * vec + blocks + as + am + pm + pk + 3 * sp + slotc + pin + 6 * per + amask + cst + 256;
     if (hipMalloc(&b->arena, total) != hipSuccess) {
         delete b;
         return set_error(OC_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(total) + " bytes failed");
@@ -347,6 +348,7 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
     d.queue = (int*)take(256);
     d.asm_mask = (const uint32_t*)take(amask);
     d.stencil_asm = st_asm ? 1 : 0;
+    d.cst = (float*)take(cst);
 
     auto fail_free = [&](int code, const std::string& msg) {
         (void)hipFree(b->arena);
@@ -410,6 +412,7 @@ int oc_batch_create(int32_t n_cloths, const int32_t* nx, const int32_t* ny, cons
             hipMemcpy((void*)d.inst_n, nv.data(), sizeof(int) * n_cloths, hipMemcpyHostToDevice) != hipSuccess)
             return fail_free(OC_ERR_DEVICE, "upload failed");
         if (oc::launch_batch_slot_constants(d, 0, 3, nullptr) != hipSuccess ||
+            oc::launch_batch_owner_constants(d, nullptr) != hipSuccess ||
             hipDeviceSynchronize() != hipSuccess)
             return fail_free(OC_ERR_DEVICE, "slot constants failed");
     }
@@ -623,6 +626,7 @@ int oc_batch_set_spring_stiffness(oc_batch b, const float* ks) {
     OC_HIP(hipStreamSynchronize(b->stream));
     OC_HIP(hipMemcpy((void*)b->dc.ks, ks, sizeof(float) * b->Stot, hipMemcpyHostToDevice));
     OC_HIP(oc::launch_batch_slot_constants(b->dc, 1, 1, b->stream));
+    OC_HIP(oc::launch_batch_owner_constants(b->dc, b->stream));
     OC_HIP(hipStreamSynchronize(b->stream));
     b->roll_valid = false;
     return OC_OK;

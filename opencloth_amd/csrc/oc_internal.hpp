@@ -64,6 +64,11 @@ __host__ __device__ constexpr int owned_k(int o) {
 __host__ __device__ constexpr int owned_partner_k(int o) {
     return o == 0 ? 0 : o == 1 ? 2 : o == 2 ? 4 : o == 3 ? 5 : o == 4 ? 8 : 10;
 }
+// Owned direction of stencil position k: the pair's owned index, whether the
+// vertex owns it (k = owned_k(o)) or is its partner (k = owned_partner_k(o)).
+__host__ __device__ constexpr int owned_index(int k) {
+    return k <= 1 ? 0 : k <= 3 ? 1 : k == 4 || k == 7 ? 2 : k <= 6 ? 3 : k <= 9 ? 4 : 5;
+}
 __host__ __device__ constexpr int own_dx(int o) { return stencil_di(owned_k(o)); }
 __host__ __device__ constexpr int own_dy(int o) { return stencil_dj(owned_k(o)); }
 
@@ -439,6 +444,10 @@ struct DevBatch {
     // the result from LDS; lds_floats of dynamic LDS per workgroup
     int asm_once;
     int lds_floats;
+    // owner-plane constants of the stencil form: constant q (L0, ks, kd) of
+    // the first spring of the pair vertex i owns in direction o at
+    // cst[(o * 3 + q) * Ntot + i]; a partner reads its owner's entry
+    float* cst;
 };
 
 // Spring-once batched assembly: owner results of direction o for the
@@ -482,5 +491,7 @@ hipError_t launch_batch_reset(const DevBatch& c, hipStream_t st);
 hipError_t launch_batch_slot_constants(const DevBatch& c, int q0, int nq, hipStream_t st);
 hipError_t launch_batch_step(const DevBatch& c, const StepScalars& s, int block_threads,
                              int grid, hipStream_t st);
+// Fill the owner-plane constants (c.cst) from the slot-major copies.
+hipError_t launch_batch_owner_constants(const DevBatch& c, hipStream_t st);
 
 }  // namespace oc
