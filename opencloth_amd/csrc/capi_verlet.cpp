@@ -6,6 +6,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <new>
 #include <string>
 #include <vector>
@@ -34,6 +36,8 @@ struct oc_verlet_s {
     float* L0 = nullptr;
     float* ks = nullptr;
     float* kd = nullptr;
+    uint32_t* asm_mask = nullptr;  // stencil assembly form (else NULL)
+    float* cst = nullptr;
 };
 
 namespace {
@@ -54,6 +58,9 @@ oc::DevVerlet view(const oc_verlet_s* v) {
     d.L0 = v->L0;
     d.ks = v->ks;
     d.kd = v->kd;
+    d.asm_mask = v->asm_mask;
+    d.cst = v->cst;
+    d.stencil_asm = v->asm_mask ? 1 : 0;
     return d;
 }
 
@@ -119,7 +126,13 @@ int oc_verlet_create(int32_t nx, int32_t ny, const float* x0, const float* x_las
     const size_t as = align_up(sizeof(int32_t) * std::max<size_t>(t.ainc_spring.size(), 1));
     const size_t am = align_up(sizeof(uint16_t) * std::max<size_t>(t.ainc_meta.size(), 1));
     const size_t sp = align_up(sizeof(float) * S);
-    const size_t total = 4 * vec + align_up(N) + as + am + 3 * sp;
+    // stencil assembly form: springs from a per-vertex mask, constants from
+    // owner planes (OPENCLOTH_ASSEMBLY=per-end keeps the slot tables)
+    const char* asm_env = std::getenv("OPENCLOTH_ASSEMBLY");
+    const bool st_asm = t.stencil_asm && !(asm_env && std::strcmp(asm_env, "per-end") == 0);
+    const size_t amask = st_asm ? align_up(sizeof(uint32_t) * N) : 0;
+    const size_t cst = st_asm ? align_up(sizeof(float) * oc::kCstPlanes * N) : 0;
+    const size_t total = 4 * vec + align_up(N) + as + am + 3 * sp + amask + cst;
     if (hipMalloc(&v->arena, total) != hipSuccess) {
         delete v;
         return set_error(OC_ERR_OUT_OF_MEMORY, "hipMalloc failed");
@@ -142,6 +155,12 @@ int oc_verlet_create(int32_t nx, int32_t ny, const float* x0, const float* x_las
     v->ks = (float*)p;
     p += sp;
     v->kd = (float*)p;
+    p += sp;
+    if (st_asm) {
+        v->asm_mask = (uint32_t*)p;
+        p += amask;
+        v->cst = (float*)p;
+    }
     std::vector<float> L0(S, 0.0f), ks(S, 0.0f), kd(S, 0.0f);
     for (int s = 0; s < t.S; s++) {
         L0[s] = springs[s].rest_length;
@@ -158,6 +177,17 @@ int oc_verlet_create(int32_t nx, int32_t ny, const float* x0, const float* x_las
               hipMemcpy(v->ks, ks.data(), S * 4, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(v->kd, kd.data(), S * 4, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(v->pin, pins.data(), N, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && st_asm) {
+        std::vector<float> hc((size_t)oc::kCstPlanes * N, 0.0f);
+        for (int s = 0; s < t.S; s++) {
+            const size_t base = (size_t)t.spring_cst[s];
+            hc[base] = L0[s];
+            hc[base + N] = ks[s];
+            hc[base + 2 * N] = kd[s];
+        }
+        ok = hipMemcpy(v->asm_mask, t.asm_mask.data(), N * 4, hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(v->cst, hc.data(), hc.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
+    }
     if (!ok) {
         (void)hipFree(v->arena);
         delete v;

@@ -307,6 +307,11 @@ struct DevVerlet {
     const float* L0;
     const float* ks;
     const float* kd;
+    // stencil assembly form: ownership mask per vertex and owner-plane
+    // constants (the layout of DevCloth::cst); stencil_asm = 0 without it
+    const uint32_t* asm_mask;
+    const float* cst;
+    int stencil_asm;
 };
 struct VerletScalars {
     float mass, gx, gy, gz, damping, dt;

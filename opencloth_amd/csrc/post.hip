@@ -114,7 +114,11 @@ struct VerletShared {
 // evaluated from this vertex's side (x1 = x_i), which is bitwise the negated
 // p1-side force, so it is always added.  FORCES: write the force; otherwise
 // integrate (VL:428-444) into the next positions.
-template <int TX, int TY, int WA, bool FORCES>
+// STENCIL: the topology has the stencil assembly form; a vertex's springs
+// come from its ownership mask in stencil order (the slot order on such
+// topologies) and their constants from the owner planes at the pair's lower
+// vertex, so no slot table or spring-id gather is read.
+template <int TX, int TY, int WA, bool FORCES, bool STENCIL>
 __global__ __launch_bounds__(TX* TY) void k_verlet(DevVerlet c, VerletScalars s) {
     constexpr int NT = TX * TY, H = kHalo;
     constexpr int WX = TX + 2 * H, WY = TY + 2 * H;
@@ -129,10 +133,15 @@ __global__ __launch_bounds__(TX* TY) void k_verlet(DevVerlet c, VerletScalars s)
     const int N = c.N, i = gj * c.nx + gi;
     int sps[WA];
     uint16_t ms[WA];
+    uint32_t mi = 0u;
+    if constexpr (STENCIL) {
+        mi = own ? c.asm_mask[i] : 0u;
+    } else {
 #pragma unroll
-    for (int k = 0; k < WA; k++) {
-        sps[k] = (own && k < c.wa) ? c.ainc_s[k * N + i] : -1;
-        ms[k] = (own && k < c.wa) ? c.ainc_m[k * N + i] : kEmptySlot;
+        for (int k = 0; k < WA; k++) {
+            sps[k] = (own && k < c.wa) ? c.ainc_s[k * N + i] : -1;
+            ms[k] = (own && k < c.wa) ? c.ainc_m[k * N + i] : kEmptySlot;
+        }
     }
     const int x0 = bx * TX - H, y0 = by * TY - H;
 #pragma unroll
@@ -165,7 +174,22 @@ __global__ __launch_bounds__(TX* TY) void k_verlet(DevVerlet c, VerletScalars s)
     V3 F = {0.0f, 0.0f, 0.0f};
     if (!pinned) F = {F.x + s.gx * s.mass, F.y + s.gy * s.mass, F.z + s.gz * s.mass};
     F = {F.x + s.damping * vi.x, F.y + s.damping * vi.y, F.z + s.damping * vi.z};
-    if (!pinned) {
+    if (!pinned && STENCIL) {
+#pragma unroll
+        for (int k = 0; k < kStencil; k++) {
+            if (!((mi >> k) & 1u)) continue;
+            const int wy = cy + stencil_dj(k), wx = cx + stencil_di(k);
+            const V3 xj = {sh.wx[0][wy][wx], sh.wx[1][wy][wx], sh.wx[2][wy][wx]};
+            const V3 vj = {sh.wv[0][wy][wx], sh.wv[1][wy][wx], sh.wv[2][wy][wx]};
+            const int g = stencil_owner(k) ? i : i + stencil_dj(k) * c.nx + stencil_di(k);
+            const float* cp = c.cst + (size_t)owned_index(k) * 3 * N + g;
+            F = add3(F, spring_force(xi, xj, vi, vj, cp[0], cp[N], cp[2 * (size_t)N]));
+            if ((mi >> (16 + k)) & 1u) {  // the pair's second spring (multiplicity 1 planes)
+                const float* c2 = cp + (size_t)kOwned * 3 * N;
+                F = add3(F, spring_force(xi, xj, vi, vj, c2[0], c2[N], c2[2 * (size_t)N]));
+            }
+        }
+    } else if (!pinned) {
 #pragma unroll
         for (int k = 0; k < WA; k++) {
             const int sp = sps[k];
@@ -196,11 +220,13 @@ template <bool FORCES>
 hipError_t launch_verlet(const DevVerlet& c, const VerletScalars& s, hipStream_t st) {
     constexpr int TX = kVerletTileX, TY = kVerletTileY;
     const int nt = ((c.nx + TX - 1) / TX) * ((c.ny + TY - 1) / TY);
-    if (c.wa <= 16)
-        hipLaunchKernelGGL((k_verlet<TX, TY, 16, FORCES>), dim3(nt), dim3(TX * TY), 0, st, c, s);
+    if (c.stencil_asm)
+        hipLaunchKernelGGL((k_verlet<TX, TY, 1, FORCES, true>), dim3(nt), dim3(TX * TY), 0, st, c, s);
+    else if (c.wa <= 16)
+        hipLaunchKernelGGL((k_verlet<TX, TY, 16, FORCES, false>), dim3(nt), dim3(TX * TY), 0, st, c, s);
     else
-        hipLaunchKernelGGL((k_verlet<TX, TY, kMaxAssemblyWidth, FORCES>), dim3(nt), dim3(TX * TY), 0, st,
-                           c, s);
+        hipLaunchKernelGGL((k_verlet<TX, TY, kMaxAssemblyWidth, FORCES, false>), dim3(nt), dim3(TX * TY), 0,
+                           st, c, s);
     return hipGetLastError();
 }
 

@@ -101,6 +101,38 @@ def test_verlet_forces_bitwise(o32):
     assert np.array_equal(bits(F), bits(Fr))
 
 
+def test_verlet_stencil_form_equals_slot_form(o32, monkeypatch):
+    """A perturbed 64x48 grid (its spring list has the stencil assembly form,
+    duplicated bends included): forces and 20 steps through the stencil form
+    (springs from the ownership mask, constants from owner planes) equal the
+    slot-table form (OPENCLOTH_ASSEMBLY=per-end) and the oracle bit for bit."""
+    from opencloth_amd.cloth import VerletCloth
+
+    nx, ny = 64, 48
+    X0 = scenes.grid_positions(nx, ny)
+    sp = scenes.grid_springs(nx, ny, X0, ks=(40.0, 30.0, 20.0), kd=(-0.2, -0.1, -0.05))
+    rng = np.random.default_rng(5)
+    X = (X0 + 0.02 * rng.standard_normal(X0.shape)).astype(np.float32)
+    XL = (X - 0.005 * rng.standard_normal(X0.shape)).astype(np.float32)
+    pinned = np.zeros(nx * ny, np.uint8)
+    pinned[[0, nx - 1]] = 1
+    kw = dict(mass=1.0, gravity=(0.0, -9.81, 0.0), damping=-0.0125, dt=1.0 / 60.0, X_last=XL)
+    out = []
+    for form in ("stencil", "per-end"):
+        if form == "per-end":
+            monkeypatch.setenv("OPENCLOTH_ASSEMBLY", "per-end")
+        vc = VerletCloth(nx, ny, X, sp, pinned, **kw)
+        F = vc.compute_forces()
+        vc.step(20)
+        out.append((F, vc.get_state()[0]))
+        monkeypatch.delenv("OPENCLOTH_ASSEMBLY", raising=False)
+    s = o32.system(nx * ny, sp, pinned)
+    Fr = s.verlet_forces(X, XL, kw["dt"], kw["mass"], kw["gravity"], kw["damping"])
+    assert np.array_equal(bits(out[0][0]), bits(Fr))
+    assert np.array_equal(bits(out[0][0]), bits(out[1][0]))
+    assert np.array_equal(bits(out[0][1]), bits(out[1][1]))
+
+
 def test_verlet_demo_rollout(o32):
     """The Verlet demo's StepPhysics (forces, integrate, ground clamp,
     ellipsoid) for 300 steps, bit-exact."""
