@@ -64,33 +64,56 @@ __device__ inline bool provot_delta(V3 x1, V3 x2, float rest, float slack, V3& d
     return true;
 }
 
+// Every slot's loads (slot table, then the other end's position and pin and
+// the spring's rest length) are issued before the first correction is
+// computed: the pass is one short chain per vertex rather than one load
+// round trip per spring.  The corrections are then added in slot order.
+template <int WA>
 __global__ __launch_bounds__(256) void k_provot(DevCloth c, float slack) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= c.N) return;
     const int N = c.N;
+    int sps[WA];
+    uint16_t ms[WA];
+#pragma unroll
+    for (int k = 0; k < WA; k++) {
+        sps[k] = k < c.wa ? c.ainc_s[k * N + i] : -1;
+        ms[k] = k < c.wa ? c.ainc_m[k * N + i] : kEmptySlot;
+    }
     const V3 xi = ld3(c.x, N, i);
     V3 v = ld3(c.v, N, i);
     const bool pin_i = c.pin[i] != 0;
+    V3 xs[WA];
+    float rest[WA];
+    bool pins[WA];
+#pragma unroll
+    for (int k = 0; k < WA; k++) {
+        xs[k] = V3{0.0f, 0.0f, 0.0f};
+        rest[k] = 0.0f;
+        pins[k] = false;
+        if (sps[k] >= 0) {
+            const int j = i + slot_dj(ms[k]) * c.nx + slot_di(ms[k]);
+            xs[k] = ld3(c.x, N, j);
+            rest[k] = c.L0[sps[k]];
+            pins[k] = c.pin[j] != 0;
+        }
+    }
     bool touched = false;
-    for (int k = 0; k < c.wa; k++) {
-        const int sp = c.ainc_s[k * N + i];
-        if (sp < 0) break;
-        const uint16_t m = c.ainc_m[k * N + i];
-        const int j = i + slot_dj(m) * c.nx + slot_di(m);
-        const bool second = (m >> 8) & 1u;
-        const V3 xj = ld3(c.x, N, j);
+#pragma unroll
+    for (int k = 0; k < WA; k++) {
+        if (sps[k] < 0) continue;  // (the slots are packed: the rest are empty too)
+        const bool second = (ms[k] >> 8) & 1u;
         V3 dP;
-        if (!(second ? provot_delta(xj, xi, c.L0[sp], slack, dP)
-                     : provot_delta(xi, xj, c.L0[sp], slack, dP)))
+        if (!(second ? provot_delta(xs[k], xi, rest[k], slack, dP)
+                     : provot_delta(xi, xs[k], rest[k], slack, dP)))
             continue;
-        const bool pin_j = c.pin[j] != 0;
         if (!second) {  // i is p1: V[p1] -= dP unless p1 is pinned
             if (!pin_i) {
                 v = sub3(v, dP);
                 touched = true;
             }
         } else {  // i is p2: V[p2] += dP when p1 is pinned or p2 is free
-            if (pin_j || !pin_i) {
+            if (pins[k] || !pin_i) {
                 v = add3(v, dP);
                 touched = true;
             }
@@ -239,7 +262,10 @@ hipError_t launch_ellipsoid(float* x, float* w, int N, const oc_ellipsoid& e, in
 }
 
 hipError_t launch_provot(const DevCloth& c, float slack, hipStream_t st) {
-    hipLaunchKernelGGL(k_provot, dim3((c.N + 255) / 256), dim3(256), 0, st, c, slack);
+    if (c.wa <= 16)
+        hipLaunchKernelGGL(k_provot<16>, dim3((c.N + 255) / 256), dim3(256), 0, st, c, slack);
+    else
+        hipLaunchKernelGGL(k_provot<kMaxAssemblyWidth>, dim3((c.N + 255) / 256), dim3(256), 0, st, c, slack);
     return hipGetLastError();
 }
 
