@@ -12,10 +12,22 @@ state resident in HBM.  Prints one JSON line:
                stream the kernels run on
   roofline     for the dominant kernel (cg_spmv): algorithmic bytes per
                launch (88N + 32P, the spmv share of the per-iteration model
-               160N + 32P) / its average launch time (300 back-to-back
-               launches between two HIP events on the kernels' stream),
-               against 8 TB/s; traffic = HBM bytes per launch from the
-               committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+               160N + 32P) / its average launch time inside the step's launch
+               sequence (an event attached to every dispatch of the same
+               steps, oc_cloth_profile_steps), against 8 TB/s; the same over
+               300 back-to-back launches beside it; traffic = HBM bytes per
+               launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+               summary, and its fraction beside the model's
+  config       the kernel split per step, and the same workload with the
+               single-reduction CG
+  batch        4096 uneven cloths, 200 steps, one graph-replayed launch per
+               step, against one-cloth-per-launch forms; the roofline of the
+               launch after the timed steps
+  batch_adjoint, adjoint
+               batched (512 cloths, 60 steps) and single-cloth (128x128, 120
+               steps) forward rollouts and their adjoints
+  entries      Verlet, PBD and grid Verlet at 512x512, the IE demo's
+               StepPhysics at 21x21
   cpu_baseline the FP32 CPU restatement of the same step (oracle) on all
                host threads (OpenMP) and on one, each on a bounded sample of
                the same workload
@@ -339,6 +351,46 @@ def bench_batch(steps: int, warmup: int, n_cloths: int = 4096, plain_steps: int 
     }
 
 
+def bench_adjoint(steps: int = 120, every: int = 16) -> dict:
+    """Differentiable rollout of one cloth: 128x128, top edge pinned, dt 1/60,
+    CG fixed at 40 iterations, per-spring ks; loss = squared distance of the
+    final positions to a smooth target; dL/dks, dL/dx0, dL/dv0 by the
+    hand-written adjoint with a checkpoint every 16 steps (each adjoint step
+    recomputes its forward step from the checkpoints)."""
+    from opencloth_amd import scenes
+    from opencloth_amd.cloth import ImplicitCloth
+    from opencloth_amd.hiptime import Event
+
+    sc = scenes.diff_scene(steps=steps)
+    target = sc.extra["target"]
+    cl = ImplicitCloth.from_scene(sc)
+    cl.rollout_forward(steps, every)  # warm-up
+    cl.rollout_backward(2.0 * (cl.X - target))
+    cl.set_state(sc.X, sc.V)
+    e0, e1, e2 = Event(), Event(), Event()
+    e0.record()
+    cl.rollout_forward(steps, every)
+    e1.record()
+    gx = 2.0 * (cl.X - target)
+    gks, _, gv0 = cl.rollout_backward(gx)
+    e2.record()
+    e2.synchronize()
+    ms_f, ms = e0.elapsed_ms(e1), e0.elapsed_ms(e2)
+    it, st = cl.last_status()
+    N = cl.N
+    cl.close()
+    return {
+        "workload": f"{sc.nx}x{sc.ny} cloth, {len(sc.springs)} springs, {steps}-step rollout, "
+                    f"checkpoint every {every}, CG 40 iterations both directions, dt 1/60",
+        "ms_fwd_adj": ms,
+        "ms_forward": ms_f,
+        "ms_backward": ms - ms_f,
+        "vertex_steps_per_s_fwd_adj": N * steps / (ms * 1e-3),
+        "last_status": [it, st],
+        "grad_finite": bool(np.isfinite(gks).all() and np.isfinite(gv0).all()),
+    }
+
+
 def bench_batch_adjoint(n_cloths: int = 512, steps: int = 60, every: int = 10, plain: bool = True,
                         sample=(0, 255, 511)) -> dict:
     """Batched forward + adjoint: n_cloths cloths, sides uniform in [32,96],
@@ -444,7 +496,7 @@ def adjoint_kernel_profile() -> dict:
             "traffic_frac": traffic / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, "source": path[len(ROOT) + 1:]}
 
 
-LEGS = ("large", "batch", "batch_adjoint", "entries", "cpu")
+LEGS = ("large", "batch", "batch_adjoint", "adjoint", "entries", "cpu")
 
 
 def kernel_split(cl, steps: int, reps: int = 200) -> dict:
@@ -587,6 +639,8 @@ def main():
         out["batch"] = bench_batch(args.batch_steps, 2, plain=not args.no_plain)
     if "batch_adjoint" in legs:
         out["batch_adjoint"] = bench_batch_adjoint(plain=not args.no_plain)
+    if "adjoint" in legs:
+        out["adjoint"] = bench_adjoint()
     if "entries" in legs:
         out["entries"] = bench_entries()
     if "cpu" in legs:
