@@ -159,9 +159,14 @@ def bench_entries(steps: int = 100) -> dict:
     cl = ImplicitCloth.from_scene(sc)
     ell, slack = sc.extra["ellipsoid"], sc.extra["provot_slack"]
     ms = timed(lambda k: cl.step_physics(k, ell, slack), steps)
-    out["implicit_step_physics"] = {"workload": "21x21 implicit demo StepPhysics (step, collider, Provot)",
-                                    "ms_per_step": ms}
     cl.close()
+    # the same with the CG in one launch that stops at convergence (same bits)
+    p = dataclasses.replace(sc.params, cg_persistent=True)
+    cl = ImplicitCloth(sc.nx, sc.ny, sc.X, sc.springs, sc.pinned, p, sc.V)
+    ms_p = timed(lambda k: cl.step_physics(k, ell, slack), steps)
+    cl.close()
+    out["implicit_step_physics"] = {"workload": "21x21 implicit demo StepPhysics (step, collider, Provot)",
+                                    "ms_per_step": ms, "ms_per_step_persistent_cg": ms_p}
     return out
 
 
