@@ -249,7 +249,8 @@ def cpu_baseline(sc, budget_s: float = 12.0, single_budget_s: float = 6.0) -> di
     }
 
 
-def bench_batch(steps: int, warmup: int, n_cloths: int = 4096, plain_steps: int = 1, plain: bool = True) -> dict:
+def bench_batch(steps: int, warmup: int, n_cloths: int = 4096, plain_steps: int = 1, plain: bool = True,
+                schedule=None) -> dict:
     """Batched forward step: n_cloths cloths with sides uniform in [16,128],
     ks log-uniform [20,200], kd uniform [0.1,0.5], 2-8 random top-edge pins,
     CG <= 40 iterations with per-cloth relative tolerance 1e-4, dt 1/60, one
@@ -268,7 +269,11 @@ def bench_batch(steps: int, warmup: int, n_cloths: int = 4096, plain_steps: int 
     # the schedule is tuned on the state two steps in (where the timed steps
     # run), then the cloths go back to rest
     b.step(2)
-    sched = b.autotune()
+    if schedule:  # a fixed schedule (reproducible counter runs) instead of the autotuner
+        b.set_schedule(*schedule)
+        sched = schedule
+    else:
+        sched = b.autotune()
     b.set_state(sc.X, sc.V)
     b.step(warmup)
     b.synchronize()
@@ -396,7 +401,7 @@ def bench_adjoint(steps: int = 120, every: int = 16) -> dict:
     }
 
 
-def bench_batch_adjoint(n_cloths: int = 512, steps: int = 60, every: int = 10, plain: bool = True,
+def bench_batch_adjoint(n_cloths: int = 512, steps: int = 60, every: int = 10, plain: bool = True, schedule=None,
                         sample=(0, 255, 511)) -> dict:
     """Batched forward + adjoint: n_cloths cloths, sides uniform in [32,96],
     per-spring ks log-uniform [20,200], top edges pinned, 60-step rollouts to
@@ -413,7 +418,11 @@ def bench_batch_adjoint(n_cloths: int = 512, steps: int = 60, every: int = 10, p
     sc = scenes.batch_diff_scene(n_cloths=n_cloths, steps=steps)
     target = sc.extra["target"]
     b = ClothBatch.from_scene(sc)
-    sched = b.autotune()
+    if schedule:
+        b.set_schedule(*schedule)
+        sched = schedule
+    else:
+        sched = b.autotune()
     b.rollout_forward(steps, every)  # warm-up
     XT, _ = b.get_state()
     gx = (2.0 * (XT - target)).astype(np.float32)
@@ -533,6 +542,10 @@ def main():
     ap.add_argument("--no-entries", action="store_true",
                     help="skip the Verlet / PBD / StepPhysics timings")
     ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--batch-schedule", default=None, metavar="NT,VPT",
+                    type=lambda t: tuple(int(x) for x in t.split(",")),
+                    help="fixed batch schedule (block threads, vertices per thread) instead of "
+                         "the autotuner, for counter runs that must repeat one launch")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the batched legs' one-cloth-per-launch denominators (profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -641,9 +654,9 @@ def main():
             "last_status": cl1.last_status(), "kernel_ms_per_step": kernel_split(cl1, args.steps)}
         cl1.close()
     if "batch" in legs:
-        out["batch"] = bench_batch(args.batch_steps, 2, plain=not args.no_plain)
+        out["batch"] = bench_batch(args.batch_steps, 2, plain=not args.no_plain, schedule=args.batch_schedule)
     if "batch_adjoint" in legs:
-        out["batch_adjoint"] = bench_batch_adjoint(plain=not args.no_plain)
+        out["batch_adjoint"] = bench_batch_adjoint(plain=not args.no_plain, schedule=args.batch_schedule)
     if "adjoint" in legs:
         out["adjoint"] = bench_adjoint()
     if "entries" in legs:
