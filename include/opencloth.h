@@ -51,7 +51,7 @@ typedef enum {
 
 /* CG stopping tests */
 #define OC_TOL_ABSOLUTE 0 /* continue while r.r > tol^2 (IE:238) */
-#define OC_TOL_RELATIVE 1 /* continue while r.r > tol^2 * r0.r0 (BW:292) */
+#define OC_TOL_RELATIVE 1 /* continue while r.r > tol^2 * r0.r0 (BW:292, BW:319) */
 #define OC_TOL_NONE 2     /* run exactly cg_max_iters iterations */
 
 /* oc_params.flags */

@@ -66,7 +66,8 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
-HOST_SRCS = [os.path.join(HERE, "host", f) for f in ("ie_demo.cpp", "vl_demo.cpp", "pbd_demo.cpp", "vg_demo.cpp")]
+HOST_SRCS = [os.path.join(HERE, "host", f)
+             for f in ("ie_demo.cpp", "bw_demo.cpp", "vl_demo.cpp", "pbd_demo.cpp", "vg_demo.cpp")]
 
 
 def host_program(name: str) -> str:

@@ -366,11 +366,11 @@ def bw_demo_scene(n: int = 21) -> Scene:
     0.75/0.75/0.95, Kd -0.25 (BW:105-107), mass 1, gravity force -0.00981
     (BW:108-109; an acceleration at mass 1), dt 5/60 (BW:65), velocity
     damping DEFAULT_DAMPING -0.125 on every free vertex (BW:706-708), 10 CG
-    iterations to the relative tolerance 1e-3 (BW:126-128, BW:292) with the
-    Jacobi preconditioner BW:784-785 means, ground clamp (BW:790-792), pins
+    iterations to the relative tolerance 1e-3 (BW:126-128, BW:319) with the
+    Jacobi preconditioner BW:784-785 means, ground clamp (BW:794-796), pins
     {0, numX}; the post-step passes use the same collider (BW:583-586) and
     Provot slack 1.01 (BW:808).  BW's own step multiplies dv by dt twice
-    (BW:789) and leaves the damping out of its Jacobian; neither is copied."""
+    (BW:791) and leaves the damping out of its Jacobian; neither is copied."""
     X = grid_positions(n, n, 4.0, 5.0)
     springs = grid_springs(n, n, X, ks=(0.75, 0.75, 0.95), kd=(-0.25, -0.25, -0.25))
     pinned = np.zeros(n * n, np.uint8)

@@ -86,7 +86,7 @@ typedef struct {
     double mass, gx, gy, gz, h, tol;
     int32_t max_it;     /* CG iteration cap (IE:77 i_max) */
     int32_t tol_mode;   /* 0: stop when r.r <= tol^2 (IE:238); 1: when r.r <= tol^2 * r0.r0
-                           (BW:292); 2: no test, run max_it iterations */
+                           (BW:292, BW:319); 2: no test, run max_it iterations */
     int32_t warm_start; /* 1: CG starts from the previous step's dv (IE:691, BW:788) */
     int32_t dot_mode;   /* 0: sequential dots (IE:219-225); 1: tile-tree order of the
                            single-cloth HIP kernels; 2: order of the batched kernels */

@@ -1,4 +1,4 @@
-"""The C++ demo programs (opencloth_amd/host/{ie,vl,pbd,vg}_demo.cpp: each
+"""The C++ demo programs (opencloth_amd/host/{ie,bw,vl,pbd,vg}_demo.cpp: each
 reference demo's scene and StepPhysics loop through include/opencloth.hpp
 over the C ABI) against the FP32 oracle on the same scene, bit for bit, after
 enough steps for the cloth to reach the collider."""
@@ -50,6 +50,28 @@ def run_demo(name, steps, tmp_path):
     assert r.returncode == 0, r.stderr
     say(r.stdout.strip())
     return open(out, "rb").read()
+
+
+def test_cpp_bw_demo_matches_oracle(tmp_path):
+    """The Baraff-Witkin demo program (velocity damping, Jacobi PCG with the
+    relative stopping test, collider, Provot slack 1.01), 1200 steps."""
+    from oracle import Oracle
+
+    steps = 1200
+    b = run_demo("bw_demo", steps, tmp_path)
+    n, _ = struct.unpack("<ii", b[:8])
+    Xg = np.frombuffer(b, np.float32, 3 * n, 8).reshape(n, 3)
+    Vg = np.frombuffer(b, np.float32, 3 * n, 8 + 12 * n).reshape(n, 3)
+    sc = scenes.bw_demo_scene()
+    ell, slack = sc.extra["ellipsoid"], sc.extra["provot_slack"]
+    s = Oracle(32).system(sc.n_vertices, sc.springs, sc.pinned)
+    prm = oracle_params(sc.params, sc.nx, sc.ny)
+    X, V, DV = sc.X.copy(), sc.V.copy(), np.zeros_like(sc.X)
+    for _ in range(steps):
+        s.step_physics(X, V, DV, prm, ell, slack)
+    assert X[:, 1].min() < 1.0  # fell past the collider's top
+    assert np.array_equal(Xg.view(np.uint32), X.view(np.uint32))
+    assert np.array_equal(Vg.view(np.uint32), V.view(np.uint32))
 
 
 def test_cpp_verlet_demo_matches_oracle(tmp_path):

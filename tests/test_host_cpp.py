@@ -1,5 +1,5 @@
 """The C++ host layer (include/opencloth.hpp) and the demos built on it
-(opencloth_amd/host/{ie,vl,pbd,vg}_demo.cpp): each program's scene, made by its
+(opencloth_amd/host/{ie,bw,vl,pbd,vg}_demo.cpp): each program's scene, made by its
 own C++ restatement of the reference demo's InitGL, equals the Python demo
 scene bit for bit, so the GPU tests of their StepPhysics loops run the
 oracle on the same inputs (CPU only: --scene-only makes no GPU call)."""
@@ -38,6 +38,20 @@ def test_demo_scene_matches(tmp_path):
     assert np.array_equal(ell[:16].view(np.uint32), e["M"].view(np.uint32))
     assert np.array_equal(ell[16:32].view(np.uint32), e["Minv"].view(np.uint32))
     assert np.array_equal(ell[32:35], e["center"]) and ell[35] == e["radius"]
+
+
+def test_bw_demo_scene_matches(tmp_path):
+    out = tmp_path / "scene.bin"
+    r = subprocess.run([host_program("bw_demo"), "--scene-only", "--out", str(out)],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    X, springs, pinned, ell = read_scene(out)
+    sc = scenes.bw_demo_scene()
+    e = sc.extra["ellipsoid"]
+    assert np.array_equal(X.view(np.uint32), sc.X.view(np.uint32))
+    assert np.array_equal(springs.view(np.uint8), sc.springs.view(np.uint8))
+    assert np.array_equal(pinned, sc.pinned)
+    assert np.array_equal(ell[:16].view(np.uint32), e["M"].view(np.uint32))
 
 
 def test_verlet_demo_scene_matches(tmp_path):
