@@ -13,7 +13,9 @@
 // Spring constants come from slot-major copies and, when every cloth has
 // the stencil form, the SpMV takes its neighbours from a 12-bit mask.
 // Reductions: thread t of a cloth's T threads sums vertices t, t+T, ... in
-// order, then the 64-lane wave tree, then the cloth's waves in order.
+// order, then the 64-lane wave tree, then the cloth's waves in order.  A
+// thread's first kResidentPerThread vertices keep their CG vectors (d, r, q
+// and the solution) in LDS through the solve; the rest stream from memory.
 #include "oc_device.hpp"
 
 namespace oc {
@@ -42,6 +44,38 @@ __device__ inline float seg_sum(float t, int T, float* ws) {
 // positive and finite (as in the single-cloth kernels).
 __device__ inline float jacobi_inv(float dg) {
     return (dg > 0.0f && isfinite(dg)) ? 1.0f / dg : 1.0f;
+}
+
+// CG start of vertex i from its assembled sums: b = h (f + h K v),
+// r = S (b - A dv), the Jacobi entries and d = z = P^-1 r (z = r without the
+// preconditioner); returns r.z.  keep (a vertex whose CG state stays in
+// LDS): r, z and the solution's start go there, and r is not written to
+// memory (d is: the neighbours' SpMV reads it).
+__device__ inline float cg_start(const DevBatch& c, const StepScalars& s, int i, bool pinned, V3 acc, V3 kv,
+                                 V3 adv, V3 dg, V3 dvi, V3* keep) {
+    const int N = c.Ntot;
+    const float h = s.h;
+    V3 b;
+    b.x = pinned ? 0.0f : h * (acc.x + h * kv.x);
+    b.y = pinned ? 0.0f : h * (acc.y + h * kv.y);
+    b.z = pinned ? 0.0f : h * (acc.z + h * kv.z);
+    st3(c.b, N, i, b);
+    const V3 r = pinned ? V3{0.0f, 0.0f, 0.0f} : sub3(b, adv);
+    V3 z = r;
+    if (s.flags & OC_FLAG_JACOBI) {
+        const V3 pv = {jacobi_inv(dg.x), jacobi_inv(dg.y), jacobi_inv(dg.z)};
+        st3(c.pinv, N, i, pv);
+        z = {pv.x * r.x, pv.y * r.y, pv.z * r.z};
+    }
+    st3(c.d, N, i, z);
+    if (keep) {
+        keep[0] = r;
+        keep[1] = z;
+        keep[2] = dvi;
+    } else {
+        st3(c.r, N, i, r);
+    }
+    return dot3(r, z);
 }
 
 // Slot k of vertex i: spring id and meta, the other end's x, v (dv for a
@@ -77,7 +111,8 @@ __device__ inline BSpringIn fetch_bspring(const DevBatch& c, const StepScalars& 
 // Assembly of vertex i (global index) of a cloth with row length nx:
 // pair blocks it owns, rhs b, CG start r = S(b - A dv), d = z = P^-1 r (z = r
 // without the Jacobi preconditioner); returns r.z.
-__device__ inline float assemble_vertex(const DevBatch& c, const StepScalars& s, int i, int nx) {
+__device__ inline float assemble_vertex(const DevBatch& c, const StepScalars& s, int i, int nx,
+                                       V3* keep = nullptr) {
     const int N = c.Ntot;
     const float h = s.h, hh = s.hh;
     const bool pinned = c.pin[i] != 0;
@@ -121,22 +156,7 @@ __device__ inline float assemble_vertex(const DevBatch& c, const StepScalars& s,
             for (int q = 0; q < 6; q++) C[q] = 0.0f;
         }
     }
-    V3 b;
-    b.x = pinned ? 0.0f : h * (acc.x + h * kv.x);
-    b.y = pinned ? 0.0f : h * (acc.y + h * kv.y);
-    b.z = pinned ? 0.0f : h * (acc.z + h * kv.z);
-    st3(c.b, N, i, b);
-    V3 r = pinned ? V3{0.0f, 0.0f, 0.0f} : sub3(b, adv);
-    st3(c.r, N, i, r);
-    if (s.flags & OC_FLAG_JACOBI) {
-        const V3 pv = {jacobi_inv(dg.x), jacobi_inv(dg.y), jacobi_inv(dg.z)};
-        st3(c.pinv, N, i, pv);
-        const V3 z = {pv.x * r.x, pv.y * r.y, pv.z * r.z};
-        st3(c.d, N, i, z);
-        return dot3(r, z);
-    }
-    st3(c.d, N, i, r);
-    return dot3(r, r);
+    return cg_start(c, s, i, pinned, acc, kv, adv, dg, dvi, keep);
 }
 
 // Stencil form of assemble_vertex, for batches whose cloths all have the
@@ -173,7 +193,8 @@ __device__ inline StencilIn fetch_stencil(const DevBatch& c, const StepScalars& 
     return r;
 }
 
-__device__ inline float assemble_vertex_stencil(const DevBatch& c, const StepScalars& s, int i, int nx) {
+__device__ inline float assemble_vertex_stencil(const DevBatch& c, const StepScalars& s, int i, int nx,
+                                               V3* keep = nullptr) {
     const int N = c.Ntot;
     const float h = s.h, hh = s.hh;
     const bool pinned = c.pin[i] != 0;
@@ -224,22 +245,7 @@ __device__ inline float assemble_vertex_stencil(const DevBatch& c, const StepSca
         adv = add3(adv, symmv(C, sub3(in.dv, dvi)));
         dg = {dg.x - C[0], dg.y - C[1], dg.z - C[2]};
     }
-    V3 b;
-    b.x = pinned ? 0.0f : h * (acc.x + h * kv.x);
-    b.y = pinned ? 0.0f : h * (acc.y + h * kv.y);
-    b.z = pinned ? 0.0f : h * (acc.z + h * kv.z);
-    st3(c.b, N, i, b);
-    V3 r = pinned ? V3{0.0f, 0.0f, 0.0f} : sub3(b, adv);
-    st3(c.r, N, i, r);
-    if (s.flags & OC_FLAG_JACOBI) {
-        const V3 pv = {jacobi_inv(dg.x), jacobi_inv(dg.y), jacobi_inv(dg.z)};
-        st3(c.pinv, N, i, pv);
-        const V3 z = {pv.x * r.x, pv.y * r.y, pv.z * r.z};
-        st3(c.d, N, i, z);
-        return dot3(r, z);
-    }
-    st3(c.d, N, i, r);
-    return dot3(r, r);
+    return cg_start(c, s, i, pinned, acc, kv, adv, dg, dvi, keep);
 }
 
 // z = P^-1 r of vertex i (r itself without the Jacobi preconditioner).
@@ -292,16 +298,15 @@ __device__ inline V3 spmv_stencil(const DevBatch& c, int i, int nx, V3 di, V3 ac
     return acc;
 }
 
-// q_i = S (A d)_i; returns d_i . q_i.  Pair metas are loaded two slots
-// ahead and block data one slot ahead of the slot being summed.
-__device__ inline float spmv_vertex(const DevBatch& c, const StepScalars& s, int i, int nx) {
+// q_i = S (A d)_i into q; returns d_i . q_i.  Pair metas are loaded two
+// slots ahead and block data one slot ahead of the slot being summed.
+__device__ inline float spmv_core(const DevBatch& c, const StepScalars& s, int i, int nx, V3& q) {
     const int N = c.Ntot;
     const V3 di = ld3(c.d, N, i);
     V3 acc = scl3(s.md, di);
     if (c.stencil) {
         acc = spmv_stencil(c, i, nx, di, acc);
-        const V3 q = c.pin[i] ? V3{0.0f, 0.0f, 0.0f} : acc;
-        st3(c.q, N, i, q);
+        q = c.pin[i] ? V3{0.0f, 0.0f, 0.0f} : acc;
         return dot3(di, q);
     }
     const uint16_t m0 = c.wp > 0 ? c.pair_m[i] : kEmptySlot;
@@ -314,23 +319,36 @@ __device__ inline float spmv_vertex(const DevBatch& c, const StepScalars& s, int
         acc = add3(acc, symmv(cur.C, sub3(cur.dj, di)));
         cur = nxt;
     }
-    const V3 q = c.pin[i] ? V3{0.0f, 0.0f, 0.0f} : acc;
-    st3(c.q, N, i, q);
+    q = c.pin[i] ? V3{0.0f, 0.0f, 0.0f} : acc;
     return dot3(di, q);
+}
+
+// spmv_core with q_i to memory.
+__device__ inline float spmv_vertex(const DevBatch& c, const StepScalars& s, int i, int nx) {
+    V3 q;
+    const float t = spmv_core(c, s, i, nx, q);
+    st3(c.q, c.Ntot, i, q);
+    return t;
 }
 
 // Adjoint start for vertex i: gvt = gv' + h gx', r = S gvt, d = P^-1 r (the
 // solution buffer, c.dv, was zeroed by the cold assembly); returns r.(P^-1 r).
 __device__ inline float adjoint_start_vertex(const DevBatch& c, const StepScalars& s,
-                                             const BatchAdjBufs& a, int i) {
+                                             const BatchAdjBufs& a, int i, V3* keep = nullptr) {
     const int N = c.Ntot;
     const V3 gx = ld3(a.gx_out, N, i), gv = ld3(a.gv_out, N, i);
     const V3 g = {gv.x + s.h * gx.x, gv.y + s.h * gx.y, gv.z + s.h * gx.z};
     st3(a.gvt, N, i, g);
     const V3 r = c.pin[i] ? V3{0.0f, 0.0f, 0.0f} : g;
-    st3(c.r, N, i, r);
     const V3 z = precond(c, s, i, r);  // the preconditioner of the cold assembly
     st3(c.d, N, i, z);
+    if (keep) {  // as in cg_start; the solve starts from mu = 0
+        keep[0] = r;
+        keep[1] = z;
+        keep[2] = V3{0.0f, 0.0f, 0.0f};
+    } else {
+        st3(c.r, N, i, r);
+    }
     return dot3(r, z);
 }
 
@@ -552,11 +570,12 @@ __device__ inline void adjoint_gather_vertex(const DevBatch& c, const StepScalar
 template <int NT, bool ADJ, bool ONCE>
 __global__ __launch_bounds__(NT, 4) void k_batch_step(DevBatch c, StepScalars s, BatchAdjBufs adj) {
     constexpr int NW = NT / 64;
+    constexpr int RV = ONCE ? 0 : kResidentPerThread;
     __shared__ float ws[NW];
     __shared__ float sd[NW], sd_old[NW], sd0[NW], ssig[NW];
     __shared__ int sdone[NW], sit[NW], sst[NW];
     __shared__ int sitem;
-    extern __shared__ float dyn_lds[];  // spring-once assembly: lds_floats
+    extern __shared__ float dyn_lds[];  // spring-once assembly: lds_floats; else the CG's resident state
     const int tid = threadIdx.x;
     const int N = c.Ntot;
     for (;;) {
@@ -576,6 +595,23 @@ __global__ __launch_bounds__(NT, 4) void k_batch_step(DevBatch c, StepScalars s,
             Ni = c.inst_n[ins];
         }
 
+        // ---- this thread's first RV vertices keep d, r, q and the solution
+        // in LDS (slot u of the thread) for the whole solve: only d goes to
+        // memory each iteration (the neighbours' SpMV reads it there).  Same
+        // arithmetic in the same order as the vertices kept in memory.
+        constexpr int RP = RV * NT;  // floats per LDS plane: d, r, solution, q (xyz each)
+        float* const rs = dyn_lds;
+        auto rld = [&](int pl, int u) {
+            const int o = u * NT + tid;
+            return V3{rs[(3 * pl) * RP + o], rs[(3 * pl + 1) * RP + o], rs[(3 * pl + 2) * RP + o]};
+        };
+        auto rst = [&](int pl, int u, V3 v) {
+            const int o = u * NT + tid;
+            rs[(3 * pl) * RP + o] = v.x;
+            rs[(3 * pl + 1) * RP + o] = v.y;
+            rs[(3 * pl + 2) * RP + o] = v.z;
+        };
+
         // ---- assembly and CG start
         float t = 0.0f;
         if constexpr (ONCE) {
@@ -589,13 +625,22 @@ __global__ __launch_bounds__(NT, 4) void k_batch_step(DevBatch c, StepScalars s,
             t = assemble_once<NT, ADJ>(c, s, adj, dyn_lds, T, g, lt, act, voff, nx, Ni, Nmax,
                                        ring_floats(nxm));
         } else {
-            for (int vv = lt; vv < Ni; vv += T) {
-                const float rr = c.stencil_asm ? assemble_vertex_stencil(c, s, voff + vv, nx)
-                                               : assemble_vertex(c, s, voff + vv, nx);
+            // the thread's first RV vertices keep their CG state in LDS
+            int u = 0;
+            for (int vv = lt; vv < Ni; vv += T, u++) {
+                V3 keep[3];
+                V3* const kp = u < RV ? keep : nullptr;
+                const float rr = c.stencil_asm ? assemble_vertex_stencil(c, s, voff + vv, nx, ADJ ? nullptr : kp)
+                                               : assemble_vertex(c, s, voff + vv, nx, ADJ ? nullptr : kp);
                 if constexpr (ADJ)
-                    t += adjoint_start_vertex(c, s, adj, voff + vv);
+                    t += adjoint_start_vertex(c, s, adj, voff + vv, kp);
                 else
                     t += rr;
+                if (kp) {
+                    rst(0, u, keep[1]);
+                    rst(1, u, keep[0]);
+                    rst(2, u, keep[2]);
+                }
             }
         }
         float tot = seg_sum<NW>(t, T, ws);
@@ -608,6 +653,7 @@ __global__ __launch_bounds__(NT, 4) void k_batch_step(DevBatch c, StepScalars s,
             sst[g] = 0;
         }
         __syncthreads();
+
 
         // ---- filtered CG, converged cloths masked
         for (int k = 0; k < s.max_it; k++) {
@@ -625,7 +671,18 @@ __global__ __launch_bounds__(NT, 4) void k_batch_step(DevBatch c, StepScalars s,
             const bool live = act && !sdone[g];
             if (live && k > 0) {
                 const float beta = sd_old[g] != 0.0f ? sd[g] / sd_old[g] : 0.0f;
-                for (int vv = lt; vv < Ni; vv += T) {
+#pragma unroll 1
+                for (int u = 0; u < RV; u++) {
+                    const int vv = lt + u * T;
+                    if (vv < Ni) {
+                        const int i = voff + vv;
+                        const V3 z = precond(c, s, i, rld(1, u)), dp = rld(0, u);
+                        const V3 dn = {z.x + beta * dp.x, z.y + beta * dp.y, z.z + beta * dp.z};
+                        rst(0, u, dn);
+                        st3(c.d, N, i, dn);
+                    }
+                }
+                for (int vv = lt + RV * T; vv < Ni; vv += T) {
                     const int i = voff + vv;
                     V3 z = precond(c, s, i, ld3(c.r, N, i)), dp = ld3(c.d, N, i);
                     st3(c.d, N, i, V3{z.x + beta * dp.x, z.y + beta * dp.y, z.z + beta * dp.z});
@@ -633,8 +690,18 @@ __global__ __launch_bounds__(NT, 4) void k_batch_step(DevBatch c, StepScalars s,
             }
             __syncthreads();
             t = 0.0f;
-            if (live)
-                for (int vv = lt; vv < Ni; vv += T) t += spmv_vertex(c, s, voff + vv, nx);
+            if (live) {
+#pragma unroll 1
+                for (int u = 0; u < RV; u++) {
+                    const int vv = lt + u * T;
+                    if (vv < Ni) {
+                        V3 q;
+                        t += spmv_core(c, s, voff + vv, nx, q);
+                        rst(3, u, q);
+                    }
+                }
+                for (int vv = lt + RV * T; vv < Ni; vv += T) t += spmv_vertex(c, s, voff + vv, nx);
+            }
             tot = seg_sum<NW>(t, T, ws);
             if (act && lt == 0) ssig[g] = tot;
             __syncthreads();
@@ -654,8 +721,22 @@ __global__ __launch_bounds__(NT, 4) void k_batch_step(DevBatch c, StepScalars s,
                 sst[g] |= OC_SOLVE_BREAKDOWN;
             }
             t = 0.0f;
-            if (live && !bd)
-                for (int vv = lt; vv < Ni; vv += T) {
+            if (live && !bd) {
+#pragma unroll 1
+                for (int u = 0; u < RV; u++) {
+                    const int vv = lt + u * T;
+                    if (vv < Ni) {
+                        const int i = voff + vv;
+                        const V3 d = rld(0, u), q = rld(3, u);
+                        V3 x = rld(2, u), r = rld(1, u);
+                        x = {x.x + alpha * d.x, x.y + alpha * d.y, x.z + alpha * d.z};
+                        r = {r.x - alpha * q.x, r.y - alpha * q.y, r.z - alpha * q.z};
+                        rst(2, u, x);
+                        rst(1, u, r);
+                        t += dot3(r, precond(c, s, i, r));
+                    }
+                }
+                for (int vv = lt + RV * T; vv < Ni; vv += T) {
                     const int i = voff + vv;
                     V3 d = ld3(c.d, N, i), q = ld3(c.q, N, i), x = ld3(c.dv, N, i),
                        r = ld3(c.r, N, i);
@@ -665,12 +746,26 @@ __global__ __launch_bounds__(NT, 4) void k_batch_step(DevBatch c, StepScalars s,
                     st3(c.r, N, i, r);
                     t += dot3(r, precond(c, s, i, r));
                 }
+            }
             tot = seg_sum<NW>(t, T, ws);
             if (live && !bd && lt == 0) {
                 sd_old[g] = sd[g];
                 sd[g] = tot;
             }
             __syncthreads();
+        }
+
+        // ---- the resident vertices' solution back to memory (the residual
+        // is not read after the solve); the forward update below reads it
+        // from LDS
+        if constexpr (RV > 0) {
+            if (act) {
+#pragma unroll 1
+                for (int u = 0; u < RV; u++) {
+                    const int vv = lt + u * T;
+                    if (vv < Ni) st3(c.dv, N, voff + vv, rld(2, u));
+                }
+            }
         }
 
         // ---- status and update
@@ -775,22 +870,22 @@ static hipError_t launch_batch(const DevBatch& c, const StepScalars& s, const Ba
                                int block_threads, int grid, hipStream_t st) {
     if (c.asm_once && (c.lds_floats < 12 * block_threads || c.lds_floats > kBatchLdsFloats))
         return hipErrorInvalidValue;
-    const size_t lds = c.asm_once ? sizeof(float) * (size_t)c.lds_floats : 0;
+    const size_t lds = sizeof(float) * (size_t)(c.asm_once ? c.lds_floats : batch_resident_floats(block_threads));
     switch (block_threads * 2 + (c.asm_once ? 1 : 0)) {
         case 512:
-            hipLaunchKernelGGL((k_batch_step<256, ADJ, false>), dim3(grid), dim3(256), 0, st, c, s, a);
+            hipLaunchKernelGGL((k_batch_step<256, ADJ, false>), dim3(grid), dim3(256), lds, st, c, s, a);
             break;
         case 513:
             hipLaunchKernelGGL((k_batch_step<256, ADJ, true>), dim3(grid), dim3(256), lds, st, c, s, a);
             break;
         case 1024:
-            hipLaunchKernelGGL((k_batch_step<512, ADJ, false>), dim3(grid), dim3(512), 0, st, c, s, a);
+            hipLaunchKernelGGL((k_batch_step<512, ADJ, false>), dim3(grid), dim3(512), lds, st, c, s, a);
             break;
         case 1025:
             hipLaunchKernelGGL((k_batch_step<512, ADJ, true>), dim3(grid), dim3(512), lds, st, c, s, a);
             break;
         case 2048:
-            hipLaunchKernelGGL((k_batch_step<1024, ADJ, false>), dim3(grid), dim3(1024), 0, st, c, s, a);
+            hipLaunchKernelGGL((k_batch_step<1024, ADJ, false>), dim3(grid), dim3(1024), lds, st, c, s, a);
             break;
         case 2049:
             hipLaunchKernelGGL((k_batch_step<1024, ADJ, true>), dim3(grid), dim3(1024), lds, st, c, s, a);

@@ -128,6 +128,9 @@ int apply_schedule(oc_batch_s* b) {
     // the kernel's static LDS (a few hundred bytes) comes on top
     const int lds_cap = std::min(oc::kBatchLdsFloats, (dev_lds - 1024) / 4);
     const bool once = b->dc.stencil_asm && b->allow_once && lds_floats <= lds_cap;
+    if (!once && oc::batch_resident_floats(NT) > lds_cap)
+        return set_error(OC_ERR_DEVICE, "the device's LDS per workgroup cannot hold the CG's resident state (" +
+                                            std::to_string(4 * oc::batch_resident_floats(NT)) + " bytes)");
     // persistent grid: as many workgroups as fit on the device at once
     int dev = 0, cus = 0;
     OC_HIP(hipGetDevice(&dev));

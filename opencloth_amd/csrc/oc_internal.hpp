@@ -477,6 +477,10 @@ __host__ __device__ inline int ring_floats(int nx) { return 12 * ring_base(kOwne
 // cloth of the work item; schedules needing more than the device allows per
 // workgroup (at most 160 KiB) keep the per-vertex form.
 constexpr int kBatchLdsFloats = 160 * 1024 / 4;
+// Batched CG: vertices per thread whose d, r, q and solution stay in LDS for
+// a whole solve (12 floats each: 36 KiB per 256 threads, 144 KiB for 1024).
+constexpr int kResidentPerThread = 3;
+constexpr int batch_resident_floats(int block_threads) { return 12 * kResidentPerThread * block_threads; }
 
 // Buffers of one batched adjoint step (SoA, stride Ntot).
 struct BatchAdjBufs {
