@@ -52,6 +52,34 @@ def test_batch_bitwise_against_oracle(o32, sched):
         assert np.array_equal(Xg.view(np.uint32), Xr.view(np.uint32)), (c, np.abs(Xg - Xr).max())
 
 
+@pytest.mark.parametrize("sched", [(1024, 1), (256, 1)])
+def test_batch_large_cloths_bitwise(o32, sched):
+    """Cloths of 200-260 vertices a side: every thread serves 40 to 260
+    vertices, the first three with their CG vectors in LDS and the rest in
+    memory; bit-exact against the oracle."""
+    from oracle import make_params
+
+    from opencloth_amd.batch import ClothBatch
+
+    sc = scenes.batch_scene(n_cloths=3, sides=(200, 260), seed=11, steps=3)
+    say("batch of large cloths", list(zip(sc.nx.tolist(), sc.ny.tolist())), "schedule", sched)
+    b = ClothBatch.from_scene(sc)
+    b.set_schedule(*sched)
+    T = b.threads_per_cloth()
+    b.step(3, graph=False)
+    X, V = b.get_state()
+    it, st = b.status()
+    for c in range(sc.n_cloths):
+        cs = sc.cloth(c)
+        s = o32.system(cs.n_vertices, cs.springs, cs.pinned)
+        prm = make_params(cs.params, dot_mode=2, tx=int(T[c]))
+        Xr, Vr, _, _, iters, str_ = s.rollout(cs.X, cs.V, prm, 3)
+        n = cs.n_vertices
+        assert it[c] == iters[-1] and st[c] == str_, (c, it[c], iters, st[c], str_)
+        assert np.array_equal(X[sc.voff[c]:sc.voff[c] + n].view(np.uint32), Xr.view(np.uint32)), c
+        assert np.array_equal(V[sc.voff[c]:sc.voff[c] + n].view(np.uint32), Vr.view(np.uint32)), c
+
+
 def test_batch_general_slot_tables(o32):
     """One cloth's springs in a shuffled order: that cloth sums its pairs in
     another order, so the whole batch runs on the general slot tables
